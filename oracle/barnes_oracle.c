@@ -1,0 +1,434 @@
+/*
+ * barnes_oracle.c -- CPU oracle for fast Barnes interpolation (TEST INFRASTRUCTURE ONLY).
+ *
+ * This file is a plain-C, FP64 restatement of the reference algorithm
+ * (MeteoSwiss fast-barnes-py 2.0.0, fastbarnes/interpolation.py).  It keeps the
+ * reference's loop order and operation order so that its results are the
+ * reference's results bit for bit, and it is built with -ffp-contract=off so that
+ * no multiply-add is fused.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * CPU-baseline leg may load it, and only as the checker: the product path
+ * (fast-barnes-py_amd/) never calls it.
+ *
+ * Function                       reference (fastbarnes/interpolation.py)
+ *   orc_normalize_values         _normalize_values            :204-212
+ *   orc_inject                   _inject_data_1d/2d/3d        :218-322
+ *   orc_accumulate_tail_array    _accumulate_tail_array       :484-533
+ *   orc_accumulate_array         _accumulate_array            :729-772
+ *   orc_convolve                 _convolve_tail_1d/2d/3d      :372-479
+ *                                _convolve_1d/2d/3d           :617-724
+ *   orc_threshold                conv_scale_factor            :389-390,424-425,472-473,634-635,669-670,717-718
+ *   orc_half_kernel_size_opt     _get_half_kernel_size_opt    :548-552
+ *   orc_tail_value               _get_tail_value              :560-569
+ *   orc_half_kernel_size         _get_half_kernel_size        :782-785
+ *   orc_sigma_effective          _get_sigma_effective         :796-803
+ *   orc_interpolate_convol       _interpolate_opt_convol      :328-367  (tail=1)
+ *                                _interpolate_convol          :575-612  (tail=0)
+ *   orc_interpolate_naive        _interpolate_naive           :861-938
+ *   orc_interpolate_radius_bf    _interpolate_radius          :809-855  (brute force instead of kd-tree)
+ *
+ * Integer powers.  The reference runs under Numba, whose scalar lowering of
+ * `float ** int` is binary exponentiation (r = 1; while e: if e&1: r *= a;
+ * e >>= 1; a *= a) and of `float ** 2` with a literal exponent is a single
+ * product.  orc_ipow() and sq() restate exactly that.  These decide only the
+ * last ulp of the NaN-mask threshold and of the tail value; which lowering
+ * Numba picks is not pinned by any reference test.
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* float ** int as Numba's int_power_impl computes it (see header). */
+static double orc_ipow(double a, int64_t e)
+{
+    double r = 1.0;
+    int invert = 0;
+    if (e < 0) { invert = 1; e = -e; }
+    while (e != 0) {
+        if (e & 1) r *= a;
+        e >>= 1;
+        a *= a;
+    }
+    return invert ? 1.0 / r : r;
+}
+
+/* int ** int with 64-bit wrap-around, as the integer power of the reference. */
+static int64_t orc_ipow_int(int64_t a, int64_t e)
+{
+    uint64_t r = 1, b = (uint64_t)a;
+    while (e > 0) {
+        if (e & 1) r *= b;
+        e >>= 1;
+        b *= b;
+    }
+    return (int64_t)r;
+}
+
+static double sq(double x) { return x * x; }
+
+/* ---- kernel parameters ------------------------------------------------- */
+
+/* _get_half_kernel_size_opt :548-552 */
+void orc_half_kernel_size_opt(int dim, const double *sigma, const double *step, int64_t num_iter, int32_t *T)
+{
+    for (int m = 0; m < dim; m++) {
+        double s = sigma[m] / step[m];
+        double r = (sqrt(1.0 + 12 * s * s / (double)num_iter) - 1.0) / 2.0;
+        T[m] = (int32_t)r;
+    }
+}
+
+/* _get_tail_value :560-569 */
+void orc_tail_value(int dim, const double *sigma, const double *step, int64_t num_iter, double *alpha)
+{
+    int32_t T[3];
+    orc_half_kernel_size_opt(dim, sigma, step, num_iter, T);
+    for (int m = 0; m < dim; m++) {
+        int64_t hks = T[m];
+        int64_t kernel_size = 2 * hks + 1;
+        double sigma_rect_sqr = (double)((hks + 1) * hks) / 3.0 * sq(step[m]);
+        double num = 0.5 * (double)kernel_size * (sq(sigma[m]) / (double)num_iter - sigma_rect_sqr);
+        double den = sq((double)(hks + 1) * step[m]) - sq(sigma[m]) / (double)num_iter;
+        alpha[m] = num / den;
+    }
+}
+
+/* _get_half_kernel_size :782-785 */
+void orc_half_kernel_size(int dim, const double *sigma, const double *step, int64_t num_iter, int32_t *T)
+{
+    double c = sqrt(3.0 / (double)num_iter);
+    for (int m = 0; m < dim; m++)
+        T[m] = (int32_t)(c * sigma[m] / step[m] + 0.5);
+}
+
+/* _get_sigma_effective :796-803 */
+void orc_sigma_effective(int dim, const double *sigma, const double *step, int64_t num_iter, double *out)
+{
+    int32_t T[3];
+    orc_half_kernel_size(dim, sigma, step, num_iter, T);
+    for (int m = 0; m < dim; m++)
+        out[m] = sqrt((double)num_iter / 3.0 * (double)T[m] * (double)(T[m] + 1)) * step[m];
+}
+
+/* conv_scale_factor: tailed :424-425 (and :389-390, :472-473); plain :669-670 (and :634-635, :717-718) */
+double orc_threshold(int tail, int dim, const double *sigma, const double *step, const int32_t *kernel_size,
+                     int64_t num_iter, const double *tail_value, double max_dist_weight)
+{
+    double prod = 1.0;
+    for (int m = 0; m < dim; m++) {
+        double p;
+        if (tail)
+            p = orc_ipow((double)kernel_size[m] + 2 * tail_value[m], num_iter);
+        else
+            p = (double)orc_ipow_int(kernel_size[m], num_iter);
+        double f = p / sqrt(2 * 3.141592653589793) / (sigma[m] / step[m]);
+        prod *= f;
+    }
+    return prod * max_dist_weight;
+}
+
+/* ---- values ------------------------------------------------------------ */
+
+/* _normalize_values :204-212; val is modified in place, offset returned */
+double orc_normalize_values(double *val, int64_t n)
+{
+    double mn = val[0], mx = val[0];
+    for (int64_t k = 1; k < n; k++) {
+        if (val[k] < mn) mn = val[k];
+        if (val[k] > mx) mx = val[k];
+    }
+    double offset = (mn + mx) / 2.0;
+    for (int64_t k = 0; k < n; k++) val[k] -= offset;
+    return offset;
+}
+
+/* ---- injection --------------------------------------------------------- */
+
+/* _inject_data_1d/2d/3d :218-322.  vg, wg are zeroed arrays of shape size[::-1]. */
+void orc_inject(int dim, double *vg, double *wg, const double *pts, const double *val, int64_t n,
+                const double *x0, const double *step, const int64_t *size)
+{
+    if (dim == 1) {
+        for (int64_t k = 0; k < n; k++) {
+            double xc = (pts[k] - x0[0]) / step[0];
+            if (xc < 0.0 || xc >= (double)(size[0] - 1)) continue;
+            int64_t xi = (int64_t)xc;
+            double xw = xc - (double)xi;
+            double w = (1.0 - xw);
+            vg[xi] += w * val[k]; wg[xi] += w;
+            w = xw;
+            vg[xi + 1] += w * val[k]; wg[xi + 1] += w;
+        }
+    } else if (dim == 2) {
+        const int64_t W = size[0];
+        for (int64_t k = 0; k < n; k++) {
+            double xc = (pts[2 * k] - x0[0]) / step[0];
+            double yc = (pts[2 * k + 1] - x0[1]) / step[1];
+            if (xc < 0.0 || yc < 0.0 || xc >= (double)(size[0] - 1) || yc >= (double)(size[1] - 1)) continue;
+            int64_t xi = (int64_t)xc, yi = (int64_t)yc;
+            double xw = xc - (double)xi, yw = yc - (double)yi;
+            double w;
+            w = (1.0 - xw) * (1.0 - yw);
+            vg[yi * W + xi] += w * val[k];             wg[yi * W + xi] += w;
+            w = xw * (1.0 - yw);
+            vg[yi * W + xi + 1] += w * val[k];         wg[yi * W + xi + 1] += w;
+            w = xw * yw;
+            vg[(yi + 1) * W + xi + 1] += w * val[k];   wg[(yi + 1) * W + xi + 1] += w;
+            w = (1.0 - xw) * yw;
+            vg[(yi + 1) * W + xi] += w * val[k];       wg[(yi + 1) * W + xi] += w;
+        }
+    } else {
+        const int64_t W = size[0], H = size[1];
+        for (int64_t k = 0; k < n; k++) {
+            double xc = (pts[3 * k] - x0[0]) / step[0];
+            double yc = (pts[3 * k + 1] - x0[1]) / step[1];
+            double zc = (pts[3 * k + 2] - x0[2]) / step[2];
+            if (xc < 0.0 || yc < 0.0 || zc < 0.0 ||
+                xc >= (double)(size[0] - 1) || yc >= (double)(size[1] - 1) || zc >= (double)(size[2] - 1))
+                continue;
+            int64_t xi = (int64_t)xc, yi = (int64_t)yc, zi = (int64_t)zc;
+            double xw = xc - (double)xi, yw = yc - (double)yi, zw = zc - (double)zi;
+#define IDX3(z, y, x) (((z) * H + (y)) * W + (x))
+            double w;
+            w = (1.0 - xw) * (1.0 - yw) * (1.0 - zw);
+            vg[IDX3(zi, yi, xi)] += w * val[k];             wg[IDX3(zi, yi, xi)] += w;
+            w = xw * (1.0 - yw) * (1.0 - zw);
+            vg[IDX3(zi, yi, xi + 1)] += w * val[k];         wg[IDX3(zi, yi, xi + 1)] += w;
+            w = xw * yw * (1.0 - zw);
+            vg[IDX3(zi, yi + 1, xi + 1)] += w * val[k];     wg[IDX3(zi, yi + 1, xi + 1)] += w;
+            w = (1.0 - xw) * yw * (1.0 - zw);
+            vg[IDX3(zi, yi + 1, xi)] += w * val[k];         wg[IDX3(zi, yi + 1, xi)] += w;
+            w = (1.0 - xw) * (1.0 - yw) * zw;
+            vg[IDX3(zi + 1, yi, xi)] += w * val[k];         wg[IDX3(zi + 1, yi, xi)] += w;
+            w = xw * (1.0 - yw) * zw;
+            vg[IDX3(zi + 1, yi, xi + 1)] += w * val[k];     wg[IDX3(zi + 1, yi, xi + 1)] += w;
+            w = xw * yw * zw;
+            vg[IDX3(zi + 1, yi + 1, xi + 1)] += w * val[k]; wg[IDX3(zi + 1, yi + 1, xi + 1)] += w;
+            w = (1.0 - xw) * yw * zw;
+            vg[IDX3(zi + 1, yi + 1, xi)] += w * val[k];     wg[IDX3(zi + 1, yi + 1, xi)] += w;
+#undef IDX3
+        }
+    }
+}
+
+/* ---- line passes ------------------------------------------------------- */
+
+/* _accumulate_tail_array :484-533.  in_arr and h_arr are both scratch of length
+ * arr_len; the function returns whichever of the two holds the result, exactly
+ * as the reference returns its ping-pong buffer. */
+double *orc_accumulate_tail_array(double *in_arr, double *h_arr, int64_t arr_len, int64_t rect_len,
+                                  int64_t num_iter, double alpha)
+{
+    const int64_t h0 = (rect_len - 1) / 2;   /* rect_len >= 1, so floor division == C division */
+    const int64_t h0_1 = h0 + 1;
+    const int64_t h1 = rect_len - h0;
+    for (int64_t it = 0; it < num_iter; it++) {
+        double accu = 0.0;
+        int64_t k;
+        for (k = -h0; k < 0; k++)                       /* phase a */
+            accu += in_arr[k + h0];
+        for (k = 0; k < h1; k++) {                      /* phase b */
+            accu += in_arr[k + h0];
+            h_arr[k] = accu + alpha * in_arr[k + h0_1];
+        }
+        for (k = h1; k < arr_len - h0_1; k++) {         /* phase c */
+            accu += (in_arr[k + h0] - in_arr[k - h1]);
+            h_arr[k] = accu + alpha * (in_arr[k - h1] + in_arr[k + h0_1]);
+        }
+        k = arr_len - h0_1;                             /* phase c' */
+        accu += (in_arr[k + h0] - in_arr[k - h1]);
+        h_arr[k] = accu + alpha * in_arr[k - h1];
+        for (k = arr_len - h0; k < arr_len; k++) {      /* phase d */
+            accu -= in_arr[k - h1];
+            h_arr[k] = accu + alpha * in_arr[k - h1];
+        }
+        double *h = in_arr; in_arr = h_arr; h_arr = h;
+    }
+    return in_arr;
+}
+
+/* _accumulate_array :729-772 */
+double *orc_accumulate_array(double *in_arr, double *h_arr, int64_t arr_len, int64_t rect_len, int64_t num_iter)
+{
+    const int64_t h0 = (rect_len - 1) / 2;
+    const int64_t h1 = rect_len - h0;
+    for (int64_t it = 0; it < num_iter; it++) {
+        double accu = 0.0;
+        int64_t k;
+        for (k = -h0; k < 0; k++)                       /* phase a */
+            accu += in_arr[k + h0];
+        for (k = 0; k < h1; k++) {                      /* phase b */
+            accu += in_arr[k + h0];
+            h_arr[k] = accu;
+        }
+        for (k = h1; k < arr_len - h0; k++) {           /* phase c */
+            accu += (in_arr[k + h0] - in_arr[k - h1]);
+            h_arr[k] = accu;
+        }
+        for (k = arr_len - h0; k < arr_len; k++) {      /* phase d */
+            accu -= in_arr[k - h1];
+            h_arr[k] = accu;
+        }
+        double *h = in_arr; in_arr = h_arr; h_arr = h;
+    }
+    return in_arr;
+}
+
+/* Convolve every line of `grid` along one axis.  Lines have length len and element
+ * stride estride; the line starting at base is walked as grid[base + i*estride].
+ * This is the `vg[j,:] = _accumulate_*(vg[j,:].copy(), h_arr, ...)` idiom of
+ * :404-418, :440-466, :649-663, :685-711. */
+static void convolve_axis(double *grid, int64_t nlines, const int64_t *bases, int64_t len, int64_t estride,
+                          int64_t rect_len, int64_t num_iter, int tail, double alpha, double *a, double *b)
+{
+    for (int64_t l = 0; l < nlines; l++) {
+        const int64_t base = bases[l];
+        for (int64_t i = 0; i < len; i++) a[i] = grid[base + i * estride];
+        double *r = tail ? orc_accumulate_tail_array(a, b, len, rect_len, num_iter, alpha)
+                         : orc_accumulate_array(a, b, len, rect_len, num_iter);
+        for (int64_t i = 0; i < len; i++) grid[base + i * estride] = r[i];
+    }
+}
+
+/* _convolve_tail_{1,2,3}d (tail=1) :372-479 and _convolve_{1,2,3}d (tail=0) :617-724.
+ * vg and wg are modified in place: all passes x, y(, z) for vg then wg per line,
+ * then wg < thr is replaced by NaN. */
+void orc_convolve(int tail, int dim, double *vg, double *wg, const double *sigma, const double *step,
+                  const int64_t *size, const int32_t *kernel_size, int64_t num_iter, const double *tail_value,
+                  double max_dist_weight)
+{
+    int64_t W = size[0], H = dim > 1 ? size[1] : 1, D = dim > 2 ? size[2] : 1;
+    int64_t maxlen = W > H ? W : H; if (D > maxlen) maxlen = D;
+    double *a = (double *)malloc(sizeof(double) * maxlen);
+    double *b = (double *)malloc(sizeof(double) * maxlen);
+    int64_t nmax = W * H > W * D ? W * H : W * D; if (H * D > nmax) nmax = H * D;
+    int64_t *bases = (int64_t *)malloc(sizeof(int64_t) * (nmax > 0 ? nmax : 1));
+    double alpha[3] = {0, 0, 0};
+    if (tail) for (int m = 0; m < dim; m++) alpha[m] = tail_value[m];
+
+    /* the reference interleaves vg and wg line by line; the lines are independent, so
+     * the per-field order below produces identical values */
+    double *fields[2] = {vg, wg};
+    /* x pass: lines (z, y) */
+    {
+        int64_t nl = 0;
+        for (int64_t z = 0; z < D; z++) for (int64_t y = 0; y < H; y++) bases[nl++] = (z * H + y) * W;
+        for (int f = 0; f < 2; f++)
+            convolve_axis(fields[f], nl, bases, W, 1, kernel_size[0], num_iter, tail, alpha[0], a, b);
+    }
+    if (dim >= 2) {   /* y pass: lines (z, x) */
+        int64_t nl = 0;
+        for (int64_t z = 0; z < D; z++) for (int64_t x = 0; x < W; x++) bases[nl++] = z * H * W + x;
+        for (int f = 0; f < 2; f++)
+            convolve_axis(fields[f], nl, bases, H, W, kernel_size[1], num_iter, tail, alpha[1], a, b);
+    }
+    if (dim >= 3) {   /* z pass: lines (y, x) */
+        int64_t nl = 0;
+        for (int64_t y = 0; y < H; y++) for (int64_t x = 0; x < W; x++) bases[nl++] = y * W + x;
+        for (int f = 0; f < 2; f++)
+            convolve_axis(fields[f], nl, bases, D, W * H, kernel_size[2], num_iter, tail, alpha[2], a, b);
+    }
+    double thr = orc_threshold(tail, dim, sigma, step, kernel_size, num_iter, alpha, max_dist_weight);
+    const int64_t total = W * H * D;
+    for (int64_t i = 0; i < total; i++)
+        if (wg[i] < thr) wg[i] = NAN;
+    free(a); free(b); free(bases);
+}
+
+/* _interpolate_opt_convol :328-367 (tail=1) and _interpolate_convol :575-612 (tail=0).
+ * out is float32 of shape size[::-1].  vg_out/wg_out, when non-NULL, receive the
+ * masked FP64 fields before the quotient (for diagnostics). */
+void orc_interpolate_convol(int tail, int dim, const double *pts, const double *val_in, int64_t n,
+                            const double *sigma, const double *x0, const double *step, const int64_t *size,
+                            int64_t num_iter, double max_dist_weight, float *out, double *vg_out, double *wg_out)
+{
+    double *val = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    memcpy(val, val_in, sizeof(double) * n);
+    double offset = orc_normalize_values(val, n);
+    int64_t total = 1;
+    for (int m = 0; m < dim; m++) total *= size[m];
+    double *vg = (double *)calloc(total, sizeof(double));
+    double *wg = (double *)calloc(total, sizeof(double));
+    orc_inject(dim, vg, wg, pts, val, n, x0, step, size);
+    int32_t T[3], K[3];
+    double alpha[3] = {0, 0, 0};
+    if (tail) {
+        orc_half_kernel_size_opt(dim, sigma, step, num_iter, T);
+        orc_tail_value(dim, sigma, step, num_iter, alpha);
+    } else {
+        orc_half_kernel_size(dim, sigma, step, num_iter, T);
+    }
+    for (int m = 0; m < dim; m++) K[m] = 2 * T[m] + 1;
+    orc_convolve(tail, dim, vg, wg, sigma, step, size, K, num_iter, alpha, max_dist_weight);
+    for (int64_t i = 0; i < total; i++) out[i] = (float)(vg[i] / wg[i] + offset);
+    if (vg_out) memcpy(vg_out, vg, sizeof(double) * total);
+    if (wg_out) memcpy(wg_out, wg, sizeof(double) * total);
+    free(vg); free(wg); free(val);
+}
+
+/* _interpolate_naive :861-938.  Sums run sequentially in point order; the
+ * reference's np.dot may use a different association (parity to a tolerance). */
+void orc_interpolate_naive(int dim, const double *pts, const double *val_in, int64_t n, const double *sigma,
+                           const double *x0, const double *step, const int64_t *size, double *out)
+{
+    double *val = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    memcpy(val, val_in, sizeof(double) * n);
+    double offset = orc_normalize_values(val, n);
+    double scale[3];
+    for (int m = 0; m < dim; m++) scale[m] = 2 * sq(sigma[m]);
+    int64_t W = size[0], H = dim > 1 ? size[1] : 1, D = dim > 2 ? size[2] : 1;
+    for (int64_t z = 0; z < D; z++) {
+        double zc = dim > 2 ? x0[2] + (double)z * step[2] : 0.0;
+        for (int64_t y = 0; y < H; y++) {
+            double yc = dim > 1 ? x0[1] + (double)y * step[1] : 0.0;
+            for (int64_t x = 0; x < W; x++) {
+                double xc = x0[0] + (double)x * step[0];
+                double ws = 0.0, wt = 0.0;
+                for (int64_t k = 0; k < n; k++) {
+                    double d = sq(pts[k * dim] - xc) / scale[0];
+                    if (dim > 1) d = d + sq(pts[k * dim + 1] - yc) / scale[1];
+                    if (dim > 2) d = d + sq(pts[k * dim + 2] - zc) / scale[2];
+                    double w = exp(-d);
+                    ws += w * val[k];
+                    wt += w;
+                }
+                out[(z * H + y) * W + x] = wt > 0.0 ? ws / wt + offset : NAN;
+            }
+        }
+    }
+    free(val);
+}
+
+/* _interpolate_radius :809-855 with the kd-tree replaced by a brute-force scan that
+ * keeps the tree's inclusion test sqr_dist <= radius^2 (util/kdtree.py:305-308).
+ * 2D only, scalar sigma. */
+void orc_interpolate_radius_bf(const double *pts, const double *val_in, int64_t n, double sigma,
+                               const double *x0, const double *step, const int64_t *size,
+                               double max_dist_weight, double min_weight, double *out)
+{
+    double *val = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    memcpy(val, val_in, sizeof(double) * n);
+    double offset = orc_normalize_values(val, n);
+    double search_radius = sqrt(-2.0 * log(min_weight)) * sigma;
+    double radius_sqr = search_radius * search_radius;
+    double scale = 2 * sq(sigma);
+    for (int64_t j = 0; j < size[1]; j++) {
+        double cy = x0[1] + (double)j * step[1];
+        for (int64_t i = 0; i < size[0]; i++) {
+            double cx = x0[0] + (double)i * step[0];
+            double ws = 0.0, wt = 0.0;
+            for (int64_t k = 0; k < n; k++) {
+                double dx = cx - pts[2 * k], dy = cy - pts[2 * k + 1];
+                double d2 = dx * dx + dy * dy;
+                if (d2 <= radius_sqr) {
+                    double w = exp(-d2 / scale);
+                    ws += w * val[k];
+                    wt += w;
+                }
+            }
+            out[j * size[0] + i] = wt >= max_dist_weight ? ws / wt + offset : NAN;
+        }
+    }
+    free(val);
+}

@@ -1,0 +1,210 @@
+"""
+CPU oracle for fast Barnes interpolation -- TEST INFRASTRUCTURE ONLY.
+
+ctypes front end of ``oracle/liboracle.so`` (built from ``barnes_oracle.c`` by
+``oracle/Makefile``), an FP64 restatement of the reference's
+``fastbarnes/interpolation.py`` that keeps its loop and operation order.  Only
+``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg
+import this module, and only to check or time against it; the product package
+never does.
+
+Every wrapper takes and returns numpy arrays with the reference's conventions
+(``size`` in (x, y, z) order, grids indexed [y, x] / [z, y, x]).
+"""
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, 'liboracle.so')
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+def build():
+    """Compiles liboracle.so with the committed Makefile."""
+    import subprocess
+    subprocess.run(['make', '-s', '-C', _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_half_kernel_size_opt.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _i32p]
+        L.orc_half_kernel_size.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _i32p]
+        L.orc_tail_value.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp]
+        L.orc_sigma_effective.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp]
+        L.orc_threshold.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _i32p, ctypes.c_int64, _dp,
+                                    ctypes.c_double]
+        L.orc_threshold.restype = ctypes.c_double
+        L.orc_normalize_values.argtypes = [_dp, ctypes.c_int64]
+        L.orc_normalize_values.restype = ctypes.c_double
+        L.orc_inject.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int64, _dp, _dp, _i64p]
+        L.orc_accumulate_tail_array.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                                ctypes.c_double]
+        L.orc_accumulate_tail_array.restype = _dp
+        L.orc_accumulate_array.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]
+        L.orc_accumulate_array.restype = _dp
+        L.orc_convolve.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _i64p, _i32p,
+                                   ctypes.c_int64, _dp, ctypes.c_double]
+        L.orc_interpolate_convol.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp, _dp,
+                                             _dp, _i64p, ctypes.c_int64, ctypes.c_double, _fp, _dp, _dp]
+        L.orc_interpolate_naive.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp, _dp, _dp, _i64p, _dp]
+        L.orc_interpolate_radius_bf.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_double, _dp, _dp, _i64p,
+                                                ctypes.c_double, ctypes.c_double, _dp]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _vec(x, dim, dtype=np.float64):
+    if isinstance(x, (list, tuple, np.ndarray)):
+        return np.ascontiguousarray(np.asarray(x, dtype=dtype).reshape(dim))
+    return np.full(dim, x, dtype=dtype)
+
+
+def _pts(pts):
+    pts = np.asarray(pts, dtype=np.float64)
+    if pts.ndim == 1:
+        pts = pts.reshape(-1, 1)
+    return np.ascontiguousarray(pts)
+
+
+# ---- scalar / parameter helpers --------------------------------------------
+
+def half_kernel_size_opt(sigma, step, num_iter, dim=1):
+    out = np.empty(dim, dtype=np.int32)
+    lib().orc_half_kernel_size_opt(dim, _d(_vec(sigma, dim)), _d(_vec(step, dim)), num_iter,
+                                   out.ctypes.data_as(_i32p))
+    return out
+
+
+def half_kernel_size(sigma, step, num_iter, dim=1):
+    out = np.empty(dim, dtype=np.int32)
+    lib().orc_half_kernel_size(dim, _d(_vec(sigma, dim)), _d(_vec(step, dim)), num_iter,
+                               out.ctypes.data_as(_i32p))
+    return out
+
+
+def tail_value(sigma, step, num_iter, dim=1):
+    out = np.empty(dim, dtype=np.float64)
+    lib().orc_tail_value(dim, _d(_vec(sigma, dim)), _d(_vec(step, dim)), num_iter, _d(out))
+    return out
+
+
+def sigma_effective(sigma, step, num_iter, dim=1):
+    out = np.empty(dim, dtype=np.float64)
+    lib().orc_sigma_effective(dim, _d(_vec(sigma, dim)), _d(_vec(step, dim)), num_iter, _d(out))
+    return out
+
+
+def threshold(tail, sigma, step, kernel_size, num_iter, tail_value_arr, max_dist_weight, dim):
+    ks = _vec(kernel_size, dim, np.int32)
+    tv = _vec(tail_value_arr if tail_value_arr is not None else 0.0, dim)
+    return lib().orc_threshold(int(tail), dim, _d(_vec(sigma, dim)), _d(_vec(step, dim)),
+                               ks.ctypes.data_as(_i32p), num_iter, _d(tv), max_dist_weight)
+
+
+# ---- line passes ------------------------------------------------------------
+
+def accumulate_tail_array(in_arr, rect_len, num_iter, alpha):
+    """Returns the num_iter-fold tailed box filter of a 1-D line (new array)."""
+    a = np.array(in_arr, dtype=np.float64, copy=True)
+    b = np.empty_like(a)
+    r = lib().orc_accumulate_tail_array(_d(a), _d(b), len(a), rect_len, num_iter, alpha)
+    return a.copy() if ctypes.addressof(r.contents) == a.ctypes.data else b.copy()
+
+
+def accumulate_array(in_arr, rect_len, num_iter):
+    a = np.array(in_arr, dtype=np.float64, copy=True)
+    b = np.empty_like(a)
+    r = lib().orc_accumulate_array(_d(a), _d(b), len(a), rect_len, num_iter)
+    return a.copy() if ctypes.addressof(r.contents) == a.ctypes.data else b.copy()
+
+
+def convolve(tail, vg, wg, sigma, step, size, kernel_size, num_iter, tail_value_arr, max_dist_weight):
+    """In-place convolution + NaN mask of vg, wg (float64, C order, shape size[::-1])."""
+    dim = len(size)
+    assert vg.dtype == np.float64 and vg.flags.c_contiguous and wg.flags.c_contiguous
+    sz = np.asarray(size, dtype=np.int64)
+    ks = _vec(kernel_size, dim, np.int32)
+    tv = _vec(tail_value_arr if tail_value_arr is not None else 0.0, dim)
+    lib().orc_convolve(int(tail), dim, _d(vg), _d(wg), _d(_vec(sigma, dim)), _d(_vec(step, dim)),
+                       sz.ctypes.data_as(_i64p), ks.ctypes.data_as(_i32p), num_iter, _d(tv), max_dist_weight)
+
+
+# ---- whole interpolations ---------------------------------------------------
+
+def _common(pts, val, sigma, x0, step, size):
+    pts = _pts(pts)
+    dim = pts.shape[1]
+    val = np.ascontiguousarray(np.asarray(val, dtype=np.float64))
+    if not isinstance(size, (list, tuple, np.ndarray)):
+        size = (size,)
+    sz = np.asarray(size, dtype=np.int64)
+    return pts, dim, val, _vec(sigma, dim), _vec(x0, dim), _vec(step, dim), sz
+
+
+def interpolate_convol(pts, val, sigma, x0, step, size, num_iter=4, max_dist=3.5, tail=True,
+                       return_fields=False):
+    """FP64 restatement of _interpolate_opt_convol (tail=True) / _interpolate_convol (tail=False)."""
+    pts, dim, val, sg, x0v, st, sz = _common(pts, val, sigma, x0, step, size)
+    mdw = math.exp(-max_dist ** 2 / 2)
+    shape = tuple(int(s) for s in sz[::-1])
+    out = np.empty(shape, dtype=np.float32)
+    vg = np.empty(shape, dtype=np.float64) if return_fields else None
+    wg = np.empty(shape, dtype=np.float64) if return_fields else None
+    lib().orc_interpolate_convol(int(tail), dim, _d(pts), _d(val), len(val), _d(sg), _d(x0v), _d(st),
+                                 sz.ctypes.data_as(_i64p), num_iter, mdw, out.ctypes.data_as(_fp),
+                                 _d(vg) if return_fields else None, _d(wg) if return_fields else None)
+    if return_fields:
+        return out, vg, wg
+    return out
+
+
+def interpolate_naive(pts, val, sigma, x0, step, size):
+    pts, dim, val, sg, x0v, st, sz = _common(pts, val, sigma, x0, step, size)
+    out = np.empty(tuple(int(s) for s in sz[::-1]), dtype=np.float64)
+    lib().orc_interpolate_naive(dim, _d(pts), _d(val), len(val), _d(sg), _d(x0v), _d(st),
+                                sz.ctypes.data_as(_i64p), _d(out))
+    return out
+
+
+def interpolate_radius(pts, val, sigma, x0, step, size, max_dist=3.5, min_weight=0.001):
+    pts, dim, val, sg, x0v, st, sz = _common(pts, val, sigma, x0, step, size)
+    assert dim == 2
+    mdw = math.exp(-max_dist ** 2 / 2)
+    out = np.empty((int(sz[1]), int(sz[0])), dtype=np.float64)
+    lib().orc_interpolate_radius_bf(_d(pts), _d(val), len(val), float(sg[0]), _d(x0v), _d(st),
+                                    sz.ctypes.data_as(_i64p), mdw, min_weight, _d(out))
+    return out
+
+
+def inject(pts, val, x0, step, size):
+    """Returns (vg, wg) after _normalize_values + _inject_data_*d, and the offset."""
+    pts = _pts(pts)
+    dim = pts.shape[1]
+    v = np.array(val, dtype=np.float64, copy=True)
+    off = lib().orc_normalize_values(_d(v), len(v))
+    if not isinstance(size, (list, tuple, np.ndarray)):
+        size = (size,)
+    sz = np.asarray(size, dtype=np.int64)
+    shape = tuple(int(s) for s in sz[::-1])
+    vg = np.zeros(shape)
+    wg = np.zeros(shape)
+    lib().orc_inject(dim, _d(vg), _d(wg), _d(pts), _d(v), len(v), _d(_vec(x0, dim)), _d(_vec(step, dim)),
+                     sz.ctypes.data_as(_i64p))
+    return vg, wg, off
