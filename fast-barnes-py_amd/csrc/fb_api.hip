@@ -1,0 +1,1009 @@
+// fb_api.hip -- host side of libfastbarnes.so: the C ABI of include/fastbarnes.h.
+//
+// Kernel parameters (T, alpha, threshold) are derived here on the host in FP64
+// with the reference's formulas (fastbarnes/interpolation.py:548-569, :782-803,
+// :424-425, :669-670); the device kernels only do the data-parallel arithmetic.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fb_launch.h"
+
+namespace fb {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+const char *last_error() { return g_err.c_str(); }
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSegMax = 63;              // maximum segment length per lane in the fast passes
+constexpr size_t kLdsBudget = 80 * 1024; // two workgroups per CU
+constexpr size_t kLdsMax = 160 * 1024;
+
+int fail(int code, const std::string &msg)
+{
+    set_error(msg);
+    return code;
+}
+
+// ---- reference parameter formulas ------------------------------------------
+
+// float ** int as the reference evaluates it under Numba (binary exponentiation)
+double ipow(double a, int64_t e)
+{
+    double r = 1.0;
+    const bool inv = e < 0;
+    if (inv) e = -e;
+    while (e != 0) {
+        if (e & 1) r *= a;
+        e >>= 1;
+        a *= a;
+    }
+    return inv ? 1.0 / r : r;
+}
+
+int64_t ipow_int(int64_t a, int64_t e)
+{
+    uint64_t r = 1, b = (uint64_t)a;
+    while (e > 0) {
+        if (e & 1) r *= b;
+        e >>= 1;
+        b *= b;
+    }
+    return (int64_t)r;
+}
+
+double sq(double x) { return x * x; }
+
+int32_t to_i32(double r)
+{
+    if (!(r == r)) return 0;
+    if (r >= 2147483647.0) return INT32_MAX;
+    if (r <= -2147483648.0) return INT32_MIN;
+    return (int32_t)r;
+}
+
+// _get_half_kernel_size_opt, interpolation.py:548-552
+int32_t half_kernel_opt(double sigma, double step, int64_t n)
+{
+    const double s = sigma / step;
+    return to_i32((std::sqrt(1.0 + 12 * s * s / (double)n) - 1.0) / 2.0);
+}
+
+// _get_half_kernel_size, interpolation.py:782-785
+int32_t half_kernel_conv(double sigma, double step, int64_t n)
+{
+    return to_i32(std::sqrt(3.0 / (double)n) * sigma / step + 0.5);
+}
+
+// _get_tail_value, interpolation.py:560-569
+double tail_value(double sigma, double step, int64_t n)
+{
+    const int64_t hks = half_kernel_opt(sigma, step, n);
+    const int64_t K = 2 * hks + 1;
+    const double sigma_rect_sqr = (double)((hks + 1) * hks) / 3.0 * sq(step);
+    return 0.5 * (double)K * (sq(sigma) / (double)n - sigma_rect_sqr) /
+           (sq((double)(hks + 1) * step) - sq(sigma) / (double)n);
+}
+
+// _get_sigma_effective, interpolation.py:796-803
+double sigma_effective(double sigma, double step, int64_t n)
+{
+    const int64_t T = half_kernel_conv(sigma, step, n);
+    return std::sqrt((double)n / 3.0 * (double)T * (double)(T + 1)) * step;
+}
+
+// conv_scale_factor, interpolation.py:424-425 (tailed) and :669-670 (plain)
+double threshold(int tail, int dim, const double *sigma, const double *step, const int32_t *K, int64_t n,
+                 const double *alpha, double mdw)
+{
+    double prod = 1.0;
+    for (int m = 0; m < dim; ++m) {
+        const double p = tail ? ipow((double)K[m] + 2 * alpha[m], n) : (double)ipow_int(K[m], n);
+        prod *= p / std::sqrt(2 * 3.141592653589793) / (sigma[m] / step[m]);
+    }
+    return prod * mdw;
+}
+
+// ---- device memory pool ----------------------------------------------------
+
+struct Pool {
+    std::mutex mu;
+    std::multimap<std::pair<int, size_t>, void *> free_blocks;
+};
+Pool &pool()
+{
+    static Pool *p = new Pool();   // never destroyed: blocks live until process exit
+    return *p;
+}
+
+size_t round_bytes(size_t b) { return (b + 255) & ~(size_t)255; }
+
+int pool_alloc(int device, size_t bytes, void **out)
+{
+    bytes = round_bytes(bytes ? bytes : 256);
+    {
+        std::lock_guard<std::mutex> g(pool().mu);
+        auto it = pool().free_blocks.find({device, bytes});
+        if (it != pool().free_blocks.end()) {
+            *out = it->second;
+            pool().free_blocks.erase(it);
+            return FB_OK;
+        }
+    }
+    hipError_t e = hipMalloc(out, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        // release cached blocks of this device and retry once
+        std::vector<void *> rel;
+        {
+            std::lock_guard<std::mutex> g(pool().mu);
+            for (auto it = pool().free_blocks.begin(); it != pool().free_blocks.end();) {
+                if (it->first.first == device) {
+                    rel.push_back(it->second);
+                    it = pool().free_blocks.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+        }
+        for (void *p : rel) (void)hipFree(p);
+        e = hipMalloc(out, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(FB_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        }
+    }
+    return FB_OK;
+}
+
+void pool_free(int device, void *p, size_t bytes)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> g(pool().mu);
+    pool().free_blocks.insert({{device, round_bytes(bytes ? bytes : 256)}, p});
+}
+
+// per-device library stream (default priority)
+hipStream_t lib_stream(int device)
+{
+    static std::mutex mu;
+    static std::map<int, hipStream_t> streams;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = streams.find(device);
+    if (it != streams.end()) return it->second;
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        s = nullptr;
+    }
+    streams[device] = s;
+    return s;
+}
+
+// lanes per line / lines per workgroup of a fast line pass
+int pass_geometry(int64_t L, int T, int &lpl, int &lines, size_t &lds)
+{
+    const int64_t need = (L + kSegMax - 1) / kSegMax;
+    lpl = 1;
+    while (lpl < need) lpl <<= 1;
+    if (lpl > kBlock)
+        return fail(FB_ERR_UNSUPPORTED, "line length " + std::to_string(L) + " exceeds the FP32C limit of " +
+                                            std::to_string(kBlock * kSegMax));
+    lines = kBlock / lpl;
+    const size_t per_line = (size_t)(L + 2 * (int64_t)T + 2) * 2 * sizeof(float);
+    while (lines > 1 && lines * per_line > kLdsBudget) {
+        lines >>= 1;
+        lpl <<= 1;
+    }
+    lds = round_bytes(lines * per_line);
+    if (lds > kLdsMax)
+        return fail(FB_ERR_UNSUPPORTED, "line of length " + std::to_string(L) + " with half kernel " +
+                                            std::to_string(T) + " does not fit the local data share");
+    return FB_OK;
+}
+
+// order work items so that blockIdx b runs tile t with consecutive tiles on one XCD
+// (workgroups are dispatched round-robin over the 8 XCDs, each with its own L2)
+template <class V>
+V xcd_order(const V &items)
+{
+    const size_t n = items.size();
+    const size_t nx = 8;
+    const size_t per = (n + nx - 1) / nx;
+    V out;
+    out.reserve(n);
+    for (size_t b = 0; b < per * nx; ++b) {
+        const size_t t = (b % nx) * per + b / nx;
+        if (t < n) out.push_back(items[t]);
+    }
+    return out;
+}
+
+struct DevBuf {
+    void *p;
+    size_t bytes;
+};
+
+}  // namespace
+}  // namespace fb
+
+using namespace fb;
+
+// ---- plan --------------------------------------------------------------------
+
+struct fb_plan {
+    int device = 0;
+    int precision = FB_PREC_FP64;
+    int nfields = 0, dim = 0, corners = 0;
+    int64_t npoints = 0, ncells = 0, nxlines = 0, nentries = 0, total_out = 0;
+    int key_bits = 8;
+    std::vector<FieldDev> fh;
+    std::vector<DevBuf> bufs;
+    size_t ws_bytes = 0;
+
+    FieldDev *d_fields = nullptr;
+    int64_t *d_pt_prefix = nullptr, *d_xline_prefix = nullptr, *d_cell_prefix = nullptr;
+    int64_t *d_lslot_prefix[3] = {nullptr, nullptr, nullptr};
+    int64_t lslots[3] = {0, 0, 0};
+    double *d_offsets = nullptr;
+    uint32_t *d_keys = nullptr, *d_vals = nullptr, *d_keys2 = nullptr, *d_vals2 = nullptr;
+    uint32_t *d_line_count = nullptr, *d_line_start = nullptr, *d_hist = nullptr, *d_scan_tmp = nullptr;
+    double *d_vg = nullptr, *d_wg = nullptr, *d_vs = nullptr, *d_ws = nullptr;
+    float *d_v = nullptr, *d_w = nullptr;
+    PassWork *d_work[3] = {nullptr, nullptr, nullptr};
+    int nwork[3] = {0, 0, 0};
+    PassField *d_pass[3] = {nullptr, nullptr, nullptr};
+    size_t lds[3] = {0, 0, 0};
+
+    int timing = 0;
+    std::vector<hipEvent_t> events;
+    std::vector<std::string> stage_names;
+    std::vector<float> stage_ms;
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    const void *g_pts = nullptr, *g_val = nullptr, *g_out = nullptr;
+    hipStream_t g_stream = nullptr;
+
+    template <class T>
+    int alloc(T **p, size_t count)
+    {
+        void *q = nullptr;
+        const size_t bytes = count * sizeof(T);
+        int rc = pool_alloc(device, bytes, &q);
+        if (rc) return rc;
+        bufs.push_back({q, bytes});
+        ws_bytes += round_bytes(bytes ? bytes : 256);
+        *p = (T *)q;
+        return FB_OK;
+    }
+    template <class T>
+    int upload(T **p, const std::vector<T> &h)
+    {
+        int rc = alloc(p, h.size() ? h.size() : 1);
+        if (rc) return rc;
+        if (!h.empty()) FB_HIP_CHECK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+        return FB_OK;
+    }
+    ~fb_plan()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        for (auto &b : bufs) pool_free(device, b.p, b.bytes);
+    }
+};
+
+namespace fb {
+namespace {
+
+bool finite_pos(double x) { return std::isfinite(x) && x > 0.0; }
+
+// validates one field and fills its device descriptor (kernel parameters included)
+int make_field(const fb_field &f, FieldDev &d, int fast)
+{
+    if (f.dim < 1 || f.dim > 3) return fail(FB_ERR_INVALID_ARGUMENT,
+                                           "Barnes interpolation supports only sample points in dimensions 1, 2 or 3");
+    if (f.method != FB_METHOD_OPTIMIZED_CONVOLUTION && f.method != FB_METHOD_CONVOLUTION)
+        return fail(FB_ERR_INVALID_ARGUMENT, "batched plans support the convolution methods only");
+    if (f.num_iter < 1) return fail(FB_ERR_INVALID_ARGUMENT, "num_iter must be >= 1");
+    if (f.n < 1) return fail(FB_ERR_INVALID_ARGUMENT, "a field needs at least one sample point");
+    std::memset(&d, 0, sizeof(d));
+    d.dim = f.dim;
+    d.tail = f.method == FB_METHOD_OPTIMIZED_CONVOLUTION;
+    d.num_iter = f.num_iter;
+    d.n = f.n;
+    d.pt_off = f.pt_offset;
+    d.out_off = f.out_offset;
+    int32_t K[3] = {1, 1, 1};
+    double prod_scale = 1.0;
+    for (int m = 0; m < 3; ++m) d.size[m] = 1;
+    for (int m = 0; m < f.dim; ++m) {
+        if (f.size[m] < 2) return fail(FB_ERR_INVALID_ARGUMENT, "grid size must be >= 2 along every axis");
+        if (!finite_pos(f.step[m]) || !finite_pos(f.sigma[m]) || !std::isfinite(f.x0[m]))
+            return fail(FB_ERR_INVALID_ARGUMENT, "sigma and step must be finite and positive, x0 finite");
+        d.size[m] = f.size[m];
+        d.x0[m] = f.x0[m];
+        d.step[m] = f.step[m];
+        d.sigma[m] = f.sigma[m];
+        d.T[m] = d.tail ? half_kernel_opt(f.sigma[m], f.step[m], f.num_iter)
+                        : half_kernel_conv(f.sigma[m], f.step[m], f.num_iter);
+        if (d.T[m] < 0) return fail(FB_ERR_INVALID_ARGUMENT, "negative half kernel size");
+        K[m] = 2 * d.T[m] + 1;
+        if ((int64_t)K[m] >= f.size[m] || d.T[m] == INT32_MAX)
+            return fail(FB_ERR_INVALID_ARGUMENT,
+                        "resulting rectangular kernel size should be smaller w.r.t. specified grid");
+        d.alpha[m] = d.tail ? tail_value(f.sigma[m], f.step[m], f.num_iter) : 0.0;
+        d.pscale[m] = fast ? std::pow(1.0 / ((double)K[m] + 2 * d.alpha[m]), (double)f.num_iter) : 1.0;
+        prod_scale *= d.pscale[m];
+    }
+    d.thr = threshold(d.tail, f.dim, d.sigma, d.step, K, f.num_iter, d.alpha, f.max_dist_weight);
+    d.thr_fast = d.thr * prod_scale;
+    return FB_OK;
+}
+
+int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<PassField> &pf, size_t &lds)
+{
+    lds = 0;
+    pf.assign(P->nfields, PassField{});
+    for (int f = 0; f < P->nfields; ++f) {
+        const FieldDev &F = P->fh[f];
+        const int64_t W = F.size[0], H = F.size[1], D = F.size[2];
+        const int64_t L = F.size[axis];
+        int lpl = 1, lines = 1;
+        size_t l = 0;
+        int rc = pass_geometry(L, F.T[axis], lpl, lines, l);
+        if (rc) return rc;
+        lds = std::max(lds, l);
+        PassField &p = pf[f];
+        p.len = L;
+        p.T = F.T[axis];
+        p.lpl = lpl;
+        p.alpha = F.alpha[axis];
+        p.pscale = F.pscale[axis];
+        std::vector<PassWork> items;
+        if (axis == 0) {
+            p.line_stride = W;
+            p.elem_stride = 1;
+            const int64_t nlines = H * D;
+            for (int64_t l0 = 0; l0 < nlines; l0 += lines)
+                items.push_back({f, (int32_t)std::min<int64_t>(lines, nlines - l0), l0 * W});
+        } else if (axis == 1) {
+            p.line_stride = 1;
+            p.elem_stride = W;
+            for (int64_t z = 0; z < D; ++z)
+                for (int64_t x0 = 0; x0 < W; x0 += lines)
+                    items.push_back({f, (int32_t)std::min<int64_t>(lines, W - x0), z * W * H + x0});
+            items = xcd_order(items);
+        } else {
+            p.line_stride = 1;
+            p.elem_stride = W * H;
+            for (int64_t c0 = 0; c0 < W * H; c0 += lines)
+                items.push_back({f, (int32_t)std::min<int64_t>(lines, W * H - c0), c0});
+            items = xcd_order(items);
+        }
+        work.insert(work.end(), items.begin(), items.end());
+    }
+    if (work.size() > (size_t)INT32_MAX) return fail(FB_ERR_UNSUPPORTED, "too many work items");
+    return FB_OK;
+}
+
+int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precision)
+{
+    *out = nullptr;
+    if (!fields || nfields < 1) return fail(FB_ERR_INVALID_ARGUMENT, "no fields");
+    if (precision != FB_PREC_FP64 && precision != FB_PREC_FP32C)
+        return fail(FB_ERR_INVALID_ARGUMENT, "unknown precision");
+    int rc0 = fast_pass_init();
+    if (rc0) return rc0;
+    fb_plan *P = new fb_plan();
+    FB_HIP_CHECK(hipGetDevice(&P->device));
+    P->precision = precision;
+    P->nfields = nfields;
+    P->dim = fields[0].dim;
+    P->corners = 1 << P->dim;
+    const int fast = precision == FB_PREC_FP32C;
+    P->fh.resize(nfields);
+    std::vector<int64_t> pt_prefix, xline_prefix, cell_prefix, lslot[3];
+    for (int f = 0; f < nfields; ++f) {
+        int rc = make_field(fields[f], P->fh[f], fast);
+        if (rc) { delete P; return rc; }
+        FieldDev &d = P->fh[f];
+        if (d.dim != P->dim) { delete P; return fail(FB_ERR_INVALID_ARGUMENT, "all fields of a plan share dim"); }
+        if (d.pt_off != P->npoints) {
+            delete P;
+            return fail(FB_ERR_INVALID_ARGUMENT, "fields must hold consecutive point ranges (pt_offset)");
+        }
+        pt_prefix.push_back(P->npoints);
+        xline_prefix.push_back(P->nxlines);
+        cell_prefix.push_back(P->ncells);
+        d.cell_off = P->ncells;
+        d.xline_off = P->nxlines;
+        const int64_t W = d.size[0], H = d.size[1], D = d.size[2];
+        const int64_t nl[3] = {H * D, W * D, W * H};
+        for (int a = 0; a < 3; ++a) {
+            lslot[a].push_back(P->lslots[a]);
+            if (a < P->dim) P->lslots[a] += 2 * nl[a];
+        }
+        P->npoints += d.n;
+        P->nxlines += H * D;
+        P->ncells += W * H * D;
+        P->total_out = std::max(P->total_out, d.out_off + W * H * D);
+    }
+    pt_prefix.push_back(P->npoints);
+    xline_prefix.push_back(P->nxlines);
+    cell_prefix.push_back(P->ncells);
+    for (int a = 0; a < 3; ++a) lslot[a].push_back(P->lslots[a]);
+    P->nentries = P->npoints * P->corners;
+    if (P->ncells >= (int64_t)kInvalidKey || P->npoints >= (int64_t)UINT32_MAX || P->nentries >= (int64_t)UINT32_MAX) {
+        delete P;
+        return fail(FB_ERR_UNSUPPORTED, "a plan holds at most 2^32-2 grid cells and 2^32-1 contributions");
+    }
+    int bits = 1;
+    while (bits < 32 && ((int64_t)1 << bits) <= P->ncells) ++bits;
+    P->key_bits = (bits + 7) / 8 * 8;
+
+    int rc = FB_OK;
+#define TRY(x) do { rc = (x); if (rc) { delete P; return rc; } } while (0)
+    TRY(P->upload(&P->d_fields, P->fh));
+    TRY(P->upload(&P->d_pt_prefix, pt_prefix));
+    TRY(P->upload(&P->d_xline_prefix, xline_prefix));
+    TRY(P->upload(&P->d_cell_prefix, cell_prefix));
+    for (int a = 0; a < 3; ++a) TRY(P->upload(&P->d_lslot_prefix[a], lslot[a]));
+    TRY(P->alloc(&P->d_offsets, nfields));
+    TRY(P->alloc(&P->d_keys, P->nentries));
+    TRY(P->alloc(&P->d_vals, P->nentries));
+    TRY(P->alloc(&P->d_keys2, P->nentries));
+    TRY(P->alloc(&P->d_vals2, P->nentries));
+    TRY(P->alloc(&P->d_line_count, P->nxlines));
+    TRY(P->alloc(&P->d_line_start, P->nxlines));
+    const int64_t nh = radix_hist_elems(P->nentries);
+    TRY(P->alloc(&P->d_hist, nh));
+    TRY(P->alloc(&P->d_scan_tmp, scan_temp_elems(std::max(nh, P->nxlines))));
+    if (!fast) {
+        TRY(P->alloc(&P->d_vg, P->ncells));
+        TRY(P->alloc(&P->d_wg, P->ncells));
+        TRY(P->alloc(&P->d_vs, P->ncells));
+        TRY(P->alloc(&P->d_ws, P->ncells));
+    } else {
+        TRY(P->alloc(&P->d_v, P->ncells));
+        TRY(P->alloc(&P->d_w, P->ncells));
+        for (int a = 0; a < P->dim; ++a) {
+            std::vector<PassWork> work;
+            std::vector<PassField> pf;
+            TRY(build_pass(P, a, work, pf, P->lds[a]));
+            P->nwork[a] = (int)work.size();
+            TRY(P->upload(&P->d_work[a], work));
+            TRY(P->upload(&P->d_pass[a], pf));
+        }
+    }
+#undef TRY
+    *out = P;
+    return FB_OK;
+}
+
+void stage_mark(fb_plan *P, hipStream_t s, const char *name)
+{
+    if (!P->timing) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) { (void)hipGetLastError(); return; }
+    (void)hipEventRecord(e, s);
+    P->events.push_back(e);
+    P->stage_names.push_back(name);
+}
+
+int plan_run(fb_plan *P, const double *pts, const double *val, float *out, hipStream_t s)
+{
+    int rc;
+    for (hipEvent_t e : P->events) (void)hipEventDestroy(e);
+    P->events.clear();
+    P->stage_names.clear();
+    stage_mark(P, s, "begin");
+    if ((rc = launch_zero(P->d_line_count, P->nxlines * 4, s))) return rc;
+    if ((rc = launch_offsets(P->d_fields, P->nfields, val, P->d_offsets, s))) return rc;
+    if ((rc = launch_contrib(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints, P->corners, pts, P->d_keys,
+                             P->d_vals, P->d_line_count, s)))
+        return rc;
+    uint32_t *ks = nullptr, *vs = nullptr;
+    if ((rc = radix_sort_pairs(P->d_keys, P->d_vals, P->d_keys2, P->d_vals2, P->nentries, P->key_bits, P->d_hist,
+                               P->d_scan_tmp, s, &ks, &vs)))
+        return rc;
+    if ((rc = scan_exclusive(P->d_line_count, P->d_line_start, P->nxlines, P->d_scan_tmp, s))) return rc;
+    stage_mark(P, s, "sort");
+    if (P->precision == FB_PREC_FP64) {
+        if ((rc = launch_zero(P->d_vg, P->ncells * 8, s))) return rc;
+        if ((rc = launch_zero(P->d_wg, P->ncells * 8, s))) return rc;
+        if ((rc = launch_exact_inject(P->d_fields, P->d_xline_prefix, P->nfields, P->nxlines, ks, vs, P->d_line_start,
+                                      P->d_line_count, pts, val, P->d_offsets, P->d_vg, P->d_wg, s)))
+            return rc;
+        stage_mark(P, s, "inject");
+        static const char *ln[3] = {"lines_x", "lines_y", "lines_z"};
+        for (int a = 0; a < P->dim; ++a) {
+            if ((rc = launch_exact_lines(P->d_fields, P->d_lslot_prefix[a], P->nfields, P->lslots[a], a, -1, P->d_vg,
+                                         P->d_wg, P->d_vs, P->d_ws, s)))
+                return rc;
+            stage_mark(P, s, ln[a]);
+        }
+        if ((rc = launch_exact_finalize(P->d_fields, P->d_cell_prefix, P->nfields, P->ncells, P->d_vg, P->d_wg,
+                                        P->d_offsets, out, s)))
+            return rc;
+        stage_mark(P, s, "finalize");
+    } else {
+        static const char *pn[3] = {"pass_x", "pass_y", "pass_z"};
+        for (int a = 0; a < P->dim; ++a) {
+            const int fin = a == P->dim - 1;
+            if ((rc = launch_fast_pass(a == 0, P->d_fields, P->d_offsets, P->d_work[a], P->nwork[a], P->d_pass[a],
+                                       kBlock, P->lds[a], ks, vs, P->d_line_start, P->d_line_count, pts, val,
+                                       P->d_v, P->d_w, out, fin, s)))
+                return rc;
+            stage_mark(P, s, pn[a]);
+        }
+    }
+    return FB_OK;
+}
+
+int check_device_ptr(const void *p, const char *what)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(FB_ERR_INVALID_ARGUMENT, std::string(what) + " is not a device pointer");
+    }
+    if (a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged)
+        return fail(FB_ERR_INVALID_ARGUMENT, std::string(what) + " is not device memory");
+    return FB_OK;
+}
+
+// scoped device buffers for the single-call entry points
+struct Scratch {
+    int device = 0;
+    std::vector<DevBuf> b;
+    template <class T>
+    int get(T **p, size_t count)
+    {
+        void *q = nullptr;
+        int rc = pool_alloc(device, count * sizeof(T), &q);
+        if (rc) return rc;
+        b.push_back({q, count * sizeof(T)});
+        *p = (T *)q;
+        return FB_OK;
+    }
+    ~Scratch()
+    {
+        for (auto &x : b) pool_free(device, x.p, x.bytes);
+    }
+};
+
+}  // namespace
+}  // namespace fb
+
+// ============================================================================
+// C ABI
+// ============================================================================
+
+extern "C" {
+
+int fb_version(void) { return 10000; }
+
+const char *fb_last_error(void) { return fb::last_error(); }
+
+int fb_device_count(int *count)
+{
+    if (!count) return fail(FB_ERR_INVALID_ARGUMENT, "count is NULL");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        n = 0;
+    }
+    *count = n;
+    return FB_OK;
+}
+
+int fb_half_kernel_size(int method, int dim, const double *sigma, const double *step, int num_iter, int32_t *out)
+{
+    if (dim < 1 || dim > 3 || !sigma || !step || !out) return fail(FB_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (num_iter < 1) return fail(FB_ERR_INVALID_ARGUMENT, "num_iter must be >= 1");
+    for (int m = 0; m < dim; ++m)
+        out[m] = method == FB_METHOD_CONVOLUTION ? half_kernel_conv(sigma[m], step[m], num_iter)
+                                                 : half_kernel_opt(sigma[m], step[m], num_iter);
+    return FB_OK;
+}
+
+int fb_tail_value(int dim, const double *sigma, const double *step, int num_iter, double *out)
+{
+    if (dim < 1 || dim > 3 || !sigma || !step || !out) return fail(FB_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (num_iter < 1) return fail(FB_ERR_INVALID_ARGUMENT, "num_iter must be >= 1");
+    for (int m = 0; m < dim; ++m) out[m] = tail_value(sigma[m], step[m], num_iter);
+    return FB_OK;
+}
+
+int fb_sigma_effective(int dim, const double *sigma, const double *step, int num_iter, double *out)
+{
+    if (dim < 1 || dim > 3 || !sigma || !step || !out) return fail(FB_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (num_iter < 1) return fail(FB_ERR_INVALID_ARGUMENT, "num_iter must be >= 1");
+    for (int m = 0; m < dim; ++m) out[m] = sigma_effective(sigma[m], step[m], num_iter);
+    return FB_OK;
+}
+
+int fb_threshold(int method, int dim, const double *sigma, const double *step, const int32_t *kernel_size,
+                 int num_iter, const double *tail_value_, double max_dist_weight, double *thr)
+{
+    if (dim < 1 || dim > 3 || !sigma || !step || !kernel_size || !thr)
+        return fail(FB_ERR_INVALID_ARGUMENT, "bad arguments");
+    const int tail = method == FB_METHOD_OPTIMIZED_CONVOLUTION;
+    if (tail && !tail_value_) return fail(FB_ERR_INVALID_ARGUMENT, "tail_value is NULL");
+    double zero[3] = {0, 0, 0};
+    *thr = threshold(tail, dim, sigma, step, kernel_size, num_iter, tail ? tail_value_ : zero, max_dist_weight);
+    return FB_OK;
+}
+
+int fb_plan_create(fb_plan **plan, const fb_field *fields, int nfields, int precision, unsigned flags)
+{
+    (void)flags;
+    if (!plan) return fail(FB_ERR_INVALID_ARGUMENT, "plan is NULL");
+    return plan_create(plan, fields, nfields, precision);
+}
+
+int fb_plan_execute(fb_plan *P, const double *d_pts, const double *d_val, float *d_out, void *stream)
+{
+    if (!P || !d_pts || !d_val || !d_out) return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument");
+    return plan_run(P, d_pts, d_val, d_out, (hipStream_t)stream);
+}
+
+int fb_plan_execute_graph(fb_plan *P, const double *d_pts, const double *d_val, float *d_out, void *stream)
+{
+    if (!P || !d_pts || !d_val || !d_out) return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument");
+    hipStream_t s = (hipStream_t)stream;
+    if (!s) return fail(FB_ERR_INVALID_ARGUMENT, "graph execution needs a non-default stream");
+    if (!P->gexec || P->g_pts != d_pts || P->g_val != d_val || P->g_out != d_out || P->g_stream != s) {
+        if (P->gexec) { (void)hipGraphExecDestroy(P->gexec); P->gexec = nullptr; }
+        if (P->graph) { (void)hipGraphDestroy(P->graph); P->graph = nullptr; }
+        const int timing = P->timing;
+        P->timing = 0;
+        FB_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = plan_run(P, d_pts, d_val, d_out, s);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &g);
+        P->timing = timing;
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return fail(FB_ERR_HIP, std::string("graph capture failed: ") + hipGetErrorString(e));
+        P->graph = g;
+        FB_HIP_CHECK(hipGraphInstantiate(&P->gexec, g, nullptr, nullptr, 0));
+        P->g_pts = d_pts; P->g_val = d_val; P->g_out = d_out; P->g_stream = s;
+    }
+    FB_HIP_CHECK(hipGraphLaunch(P->gexec, s));
+    return FB_OK;
+}
+
+int fb_plan_set_timing(fb_plan *P, int enable)
+{
+    if (!P) return fail(FB_ERR_INVALID_ARGUMENT, "plan is NULL");
+    P->timing = enable ? 1 : 0;
+    return FB_OK;
+}
+
+int fb_plan_stage_times(fb_plan *P, float *ms, int max_stages, int *nstages, char *names, int names_len)
+{
+    if (!P || !nstages) return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument");
+    const int n = P->events.size() > 1 ? (int)P->events.size() - 1 : 0;
+    *nstages = n;
+    std::string all;
+    for (int i = 0; i < n; ++i) {
+        FB_HIP_CHECK(hipEventSynchronize(P->events[i + 1]));
+        float t = 0.f;
+        FB_HIP_CHECK(hipEventElapsedTime(&t, P->events[i], P->events[i + 1]));
+        if (ms && i < max_stages) ms[i] = t;
+        if (i) all += ",";
+        all += P->stage_names[i + 1];
+    }
+    if (names && names_len > 0) {
+        std::strncpy(names, all.c_str(), names_len - 1);
+        names[names_len - 1] = 0;
+    }
+    return FB_OK;
+}
+
+int fb_plan_info(fb_plan *P, int64_t *total_points, int64_t *workspace_bytes)
+{
+    if (!P) return fail(FB_ERR_INVALID_ARGUMENT, "plan is NULL");
+    if (total_points) *total_points = P->ncells;
+    if (workspace_bytes) *workspace_bytes = (int64_t)P->ws_bytes;
+    return FB_OK;
+}
+
+int fb_plan_destroy(fb_plan *P)
+{
+    delete P;
+    return FB_OK;
+}
+
+int fb_barnes(int method, int precision, int dim, int64_t n, const double *pts, const double *val,
+              const double *sigma, const double *x0, const double *step, const int64_t *size, int num_iter,
+              double max_dist_weight, double min_weight, void *out, unsigned flags, void *stream)
+{
+    if (dim < 1 || dim > 3)
+        return fail(FB_ERR_INVALID_ARGUMENT, "Barnes interpolation supports only sample points in dimensions 1, 2 or 3");
+    if (n < 1 || !pts || !val || !sigma || !x0 || !step || !size || !out)
+        return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument or no sample points");
+    const bool on_dev = flags & FB_FLAG_DEVICE_PTRS;
+    int device = 0;
+    FB_HIP_CHECK(hipGetDevice(&device));
+    hipStream_t s = stream ? (hipStream_t)stream : lib_stream(device);
+    int64_t cells = 1;
+    for (int m = 0; m < dim; ++m) {
+        if (size[m] < 1) return fail(FB_ERR_INVALID_ARGUMENT, "grid size must be positive");
+        cells *= size[m];
+    }
+    Scratch sc;
+    sc.device = device;
+    int rc;
+    const double *d_pts = pts, *d_val = val;
+    if (on_dev) {
+        if ((rc = check_device_ptr(pts, "pts")) || (rc = check_device_ptr(val, "val")) ||
+            (rc = check_device_ptr(out, "out")))
+            return rc;
+    } else {
+        double *a, *b;
+        if ((rc = sc.get(&a, n * dim)) || (rc = sc.get(&b, n))) return rc;
+        FB_HIP_CHECK(hipMemcpyAsync(a, pts, n * dim * sizeof(double), hipMemcpyHostToDevice, s));
+        FB_HIP_CHECK(hipMemcpyAsync(b, val, n * sizeof(double), hipMemcpyHostToDevice, s));
+        d_pts = a;
+        d_val = b;
+    }
+
+    if (method == FB_METHOD_NAIVE || method == FB_METHOD_RADIUS) {
+        if (method == FB_METHOD_RADIUS && dim != 2)
+            return fail(FB_ERR_INVALID_ARGUMENT, "radius algorithm works only in 2D");
+        FieldDev F;
+        std::memset(&F, 0, sizeof(F));
+        F.dim = dim;
+        F.n = n;
+        for (int m = 0; m < 3; ++m) F.size[m] = 1;
+        for (int m = 0; m < dim; ++m) {
+            F.size[m] = size[m];
+            F.x0[m] = x0[m];
+            F.step[m] = step[m];
+            F.sigma[m] = sigma[m];
+        }
+        FieldDev *d_f;
+        double *d_off, *d_out = (double *)out;
+        if ((rc = sc.get(&d_f, 1)) || (rc = sc.get(&d_off, 1))) return rc;
+        if (!on_dev && (rc = sc.get(&d_out, cells))) return rc;
+        FB_HIP_CHECK(hipMemcpy(d_f, &F, sizeof(F), hipMemcpyHostToDevice));
+        if ((rc = launch_offsets(d_f, 1, d_val, d_off, s))) return rc;
+        double r2 = 0.0;
+        if (method == FB_METHOD_RADIUS) {
+            // kdtree.prepare_search: radius**2 with radius = sqrt(-2 log(min_weight)) * sigma
+            const double radius = std::sqrt(-2.0 * std::log(min_weight)) * sigma[0];
+            r2 = radius * radius;
+        }
+        if ((rc = launch_naive(d_f, method == FB_METHOD_RADIUS, r2, max_dist_weight, d_pts, d_val, d_off, d_out,
+                               cells, s)))
+            return rc;
+        if (!on_dev) FB_HIP_CHECK(hipMemcpyAsync(out, d_out, cells * sizeof(double), hipMemcpyDeviceToHost, s));
+        FB_HIP_CHECK(hipStreamSynchronize(s));
+        return FB_OK;
+    }
+    if (method != FB_METHOD_OPTIMIZED_CONVOLUTION && method != FB_METHOD_CONVOLUTION)
+        return fail(FB_ERR_INVALID_ARGUMENT, "encountered invalid Barnes interpolation method");
+
+    fb_field f;
+    std::memset(&f, 0, sizeof(f));
+    f.dim = dim;
+    f.method = method;
+    f.num_iter = num_iter;
+    f.n = n;
+    f.pt_offset = 0;
+    f.out_offset = 0;
+    for (int m = 0; m < 3; ++m) f.size[m] = 1;
+    for (int m = 0; m < dim; ++m) {
+        f.size[m] = size[m];
+        f.sigma[m] = sigma[m];
+        f.x0[m] = x0[m];
+        f.step[m] = step[m];
+    }
+    f.max_dist_weight = max_dist_weight;
+    fb_plan *P = nullptr;
+    if ((rc = plan_create(&P, &f, 1, precision))) return rc;
+    float *d_out = (float *)out;
+    if (!on_dev && (rc = sc.get(&d_out, cells))) { delete P; return rc; }
+    rc = plan_run(P, d_pts, d_val, d_out, s);
+    if (!rc && !on_dev) {
+        hipError_t e = hipMemcpyAsync(out, d_out, cells * sizeof(float), hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) rc = fail(FB_ERR_HIP, std::string("copy of the result failed: ") + hipGetErrorString(e));
+    }
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = fail(FB_ERR_HIP, std::string("execution failed: ") + hipGetErrorString(e));
+    }
+    delete P;
+    return rc;
+}
+
+int fb_accumulate_lines(int tail, int precision, int64_t nlines, int64_t len, int rect_len, int num_iter, double alpha,
+                        const double *in, double *out, unsigned flags, void *stream)
+{
+    if (nlines < 1 || len < 1 || rect_len < 1 || num_iter < 0 || !in || !out)
+        return fail(FB_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (rect_len >= len) return fail(FB_ERR_INVALID_ARGUMENT, "rect_len must be smaller than the line length");
+    const bool on_dev = flags & FB_FLAG_DEVICE_PTRS;
+    int device = 0;
+    FB_HIP_CHECK(hipGetDevice(&device));
+    hipStream_t s = stream ? (hipStream_t)stream : lib_stream(device);
+    Scratch sc;
+    sc.device = device;
+    int rc;
+    const int64_t cells = nlines * len;
+    double *d_io;
+    if ((rc = sc.get(&d_io, cells))) return rc;
+    FB_HIP_CHECK(hipMemcpyAsync(d_io, in, cells * sizeof(double),
+                                on_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    FieldDev F;
+    std::memset(&F, 0, sizeof(F));
+    F.dim = 2;
+    F.tail = tail ? 1 : 0;
+    F.num_iter = num_iter;
+    F.size[0] = len;
+    F.size[1] = nlines;
+    F.size[2] = 1;
+    F.T[0] = (rect_len - 1) / 2;
+    F.alpha[0] = tail ? alpha : 0.0;
+    F.pscale[0] = 1.0;
+    FieldDev *d_f;
+    if ((rc = sc.get(&d_f, 1))) return rc;
+    FB_HIP_CHECK(hipMemcpy(d_f, &F, sizeof(F), hipMemcpyHostToDevice));
+    if (precision == FB_PREC_FP64) {
+        double *d_w, *d_vs, *d_ws;
+        int64_t *d_pre;
+        if ((rc = sc.get(&d_w, cells)) || (rc = sc.get(&d_vs, cells)) || (rc = sc.get(&d_ws, cells)) ||
+            (rc = sc.get(&d_pre, 2)))
+            return rc;
+        if ((rc = launch_zero(d_w, cells * 8, s))) return rc;
+        const int64_t pre[2] = {0, 2 * nlines};
+        FB_HIP_CHECK(hipMemcpy(d_pre, pre, sizeof(pre), hipMemcpyHostToDevice));
+        if ((rc = launch_exact_lines(d_f, d_pre, 1, 2 * nlines, 0, rect_len, d_io, d_w, d_vs, d_ws, s))) return rc;
+    } else {
+        if (rect_len % 2 == 0) return fail(FB_ERR_UNSUPPORTED, "FP32C line passes need an odd rect_len");
+        if ((rc = fast_pass_init())) return rc;
+        if (num_iter < 1) return fail(FB_ERR_INVALID_ARGUMENT, "num_iter must be >= 1");
+        float *d_v, *d_w;
+        double *d_off;
+        PassWork *d_work;
+        PassField *d_pf;
+        if ((rc = sc.get(&d_v, cells)) || (rc = sc.get(&d_w, cells)) || (rc = sc.get(&d_off, 1))) return rc;
+        int lpl, lines;
+        size_t lds;
+        if ((rc = pass_geometry(len, F.T[0], lpl, lines, lds))) return rc;
+        PassField pf;
+        pf.len = len;
+        pf.line_stride = len;
+        pf.elem_stride = 1;
+        pf.T = F.T[0];
+        pf.lpl = lpl;
+        pf.alpha = F.alpha[0];
+        pf.pscale = 1.0;
+        std::vector<PassWork> work;
+        for (int64_t l0 = 0; l0 < nlines; l0 += lines)
+            work.push_back({0, (int32_t)std::min<int64_t>(lines, nlines - l0), l0 * len});
+        if ((rc = sc.get(&d_work, work.size())) || (rc = sc.get(&d_pf, 1))) return rc;
+        FB_HIP_CHECK(hipMemcpy(d_work, work.data(), work.size() * sizeof(PassWork), hipMemcpyHostToDevice));
+        FB_HIP_CHECK(hipMemcpy(d_pf, &pf, sizeof(pf), hipMemcpyHostToDevice));
+        if ((rc = launch_zero(d_off, 8, s))) return rc;
+        if ((rc = launch_f64_to_f32(d_io, d_v, cells, s))) return rc;
+        if ((rc = launch_zero(d_w, cells * 4, s))) return rc;
+        if ((rc = launch_fast_pass(false, d_f, d_off, d_work, (int)work.size(), d_pf, kBlock, lds, nullptr, nullptr,
+                                   nullptr, nullptr, nullptr, nullptr, d_v, d_w, nullptr, 0, s)))
+            return rc;
+        if ((rc = launch_f32_to_f64(d_v, d_io, cells, 1.0, s))) return rc;
+    }
+    FB_HIP_CHECK(hipMemcpyAsync(out, d_io, cells * sizeof(double),
+                                on_dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    FB_HIP_CHECK(hipStreamSynchronize(s));
+    return FB_OK;
+}
+
+int fb_convolve_grid(int tail, int precision, int dim, const int64_t *size, const int32_t *kernel_size, int num_iter,
+                     const double *tail_value_, double thr, double *vg, double *wg, unsigned flags, void *stream)
+{
+    if (dim < 1 || dim > 3 || !size || !kernel_size || !vg || !wg || num_iter < 0)
+        return fail(FB_ERR_INVALID_ARGUMENT, "bad arguments");
+    const bool on_dev = flags & FB_FLAG_DEVICE_PTRS;
+    int device = 0;
+    FB_HIP_CHECK(hipGetDevice(&device));
+    hipStream_t s = stream ? (hipStream_t)stream : lib_stream(device);
+    FieldDev F;
+    std::memset(&F, 0, sizeof(F));
+    F.dim = dim;
+    F.tail = tail ? 1 : 0;
+    F.num_iter = num_iter;
+    int64_t cells = 1;
+    for (int m = 0; m < 3; ++m) F.size[m] = 1;
+    for (int m = 0; m < dim; ++m) {
+        if (size[m] < 1) return fail(FB_ERR_INVALID_ARGUMENT, "grid size must be positive");
+        if (kernel_size[m] < 1 || kernel_size[m] % 2 == 0 || kernel_size[m] >= size[m])
+            return fail(FB_ERR_INVALID_ARGUMENT, "kernel sizes must be odd and smaller than the grid");
+        F.size[m] = size[m];
+        F.T[m] = (kernel_size[m] - 1) / 2;
+        F.alpha[m] = (tail && tail_value_) ? tail_value_[m] : 0.0;
+        F.pscale[m] = 1.0;
+        cells *= size[m];
+    }
+    F.thr = thr;
+    F.thr_fast = thr;
+    Scratch sc;
+    sc.device = device;
+    int rc;
+    FieldDev *d_f;
+    double *d_vg, *d_wg;
+    int64_t *d_cpre;
+    if ((rc = sc.get(&d_f, 1)) || (rc = sc.get(&d_vg, cells)) || (rc = sc.get(&d_wg, cells)) ||
+        (rc = sc.get(&d_cpre, 2)))
+        return rc;
+    const hipMemcpyKind kin = on_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    FB_HIP_CHECK(hipMemcpy(d_f, &F, sizeof(F), hipMemcpyHostToDevice));
+    FB_HIP_CHECK(hipMemcpyAsync(d_vg, vg, cells * 8, kin, s));
+    FB_HIP_CHECK(hipMemcpyAsync(d_wg, wg, cells * 8, kin, s));
+    const int64_t cpre[2] = {0, cells};
+    FB_HIP_CHECK(hipMemcpy(d_cpre, cpre, sizeof(cpre), hipMemcpyHostToDevice));
+    const int64_t W = F.size[0], H = F.size[1], D = F.size[2];
+    const int64_t nl[3] = {H * D, W * D, W * H};
+    if (precision == FB_PREC_FP64) {
+        double *d_vs, *d_ws;
+        int64_t *d_pre;
+        if ((rc = sc.get(&d_vs, cells)) || (rc = sc.get(&d_ws, cells)) || (rc = sc.get(&d_pre, 2))) return rc;
+        for (int a = 0; a < dim; ++a) {
+            const int64_t pre[2] = {0, 2 * nl[a]};
+            FB_HIP_CHECK(hipMemcpy(d_pre, pre, sizeof(pre), hipMemcpyHostToDevice));
+            if ((rc = launch_exact_lines(d_f, d_pre, 1, 2 * nl[a], a, -1, d_vg, d_wg, d_vs, d_ws, s))) return rc;
+            FB_HIP_CHECK(hipStreamSynchronize(s));   // d_pre is reused by the next axis
+        }
+        if ((rc = launch_mask(d_f, d_cpre, 1, cells, d_wg, 0, s))) return rc;
+    } else {
+        if (num_iter < 1) return fail(FB_ERR_INVALID_ARGUMENT, "num_iter must be >= 1");
+        if ((rc = fast_pass_init())) return rc;
+        float *d_v, *d_w;
+        double *d_off;
+        if ((rc = sc.get(&d_v, cells)) || (rc = sc.get(&d_w, cells)) || (rc = sc.get(&d_off, 1))) return rc;
+        if ((rc = launch_zero(d_off, 8, s))) return rc;
+        if ((rc = launch_f64_to_f32(d_vg, d_v, cells, s))) return rc;
+        if ((rc = launch_f64_to_f32(d_wg, d_w, cells, s))) return rc;
+        fb_plan tmp;
+        tmp.device = device;
+        tmp.nfields = 1;
+        tmp.dim = dim;
+        tmp.fh.assign(1, F);
+        for (int a = 0; a < dim; ++a) {
+            std::vector<PassWork> work;
+            std::vector<PassField> pf;
+            size_t lds = 0;
+            if ((rc = build_pass(&tmp, a, work, pf, lds))) return rc;
+            PassWork *d_work;
+            PassField *d_pf;
+            if ((rc = sc.get(&d_work, work.size())) || (rc = sc.get(&d_pf, 1))) return rc;
+            FB_HIP_CHECK(hipMemcpy(d_work, work.data(), work.size() * sizeof(PassWork), hipMemcpyHostToDevice));
+            FB_HIP_CHECK(hipMemcpy(d_pf, pf.data(), sizeof(PassField), hipMemcpyHostToDevice));
+            if ((rc = launch_fast_pass(false, d_f, d_off, d_work, (int)work.size(), d_pf, kBlock, lds, nullptr,
+                                       nullptr, nullptr, nullptr, nullptr, nullptr, d_v, d_w, nullptr, 0, s)))
+                return rc;
+            FB_HIP_CHECK(hipStreamSynchronize(s));   // host vectors above go out of scope
+        }
+        if ((rc = launch_f32_to_f64(d_v, d_vg, cells, 1.0, s))) return rc;
+        if ((rc = launch_f32_to_f64(d_w, d_wg, cells, 1.0, s))) return rc;
+        if ((rc = launch_mask(d_f, d_cpre, 1, cells, d_wg, 1, s))) return rc;
+    }
+    const hipMemcpyKind kout = on_dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    FB_HIP_CHECK(hipMemcpyAsync(vg, d_vg, cells * 8, kout, s));
+    FB_HIP_CHECK(hipMemcpyAsync(wg, d_wg, cells * 8, kout, s));
+    FB_HIP_CHECK(hipStreamSynchronize(s));
+    return FB_OK;
+}
+
+}  // extern "C"

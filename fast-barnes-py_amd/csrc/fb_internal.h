@@ -1,0 +1,81 @@
+// fb_internal.h -- shared definitions of the HIP implementation (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace fb {
+
+// Sentinel sort key of a contribution that falls outside the grid
+// (reference: the `continue` of interpolation.py:226,249,283).
+constexpr uint32_t kInvalidKey = 0xFFFFFFFFu;
+
+// Per-field descriptor as the kernels see it.  Everything is derived on the host in
+// FP64 from the reference's own formulas, except the value offset, which the
+// offset kernel computes on the device into a separate array.
+struct FieldDev {
+    int32_t dim;
+    int32_t tail;          // 1: optimized_convolution (alpha tails), 0: convolution
+    int32_t num_iter;
+    int32_t pad0;
+    int64_t n;             // points of this field
+    int64_t pt_off;        // first point (row of pts / element of val)
+    int64_t out_off;       // first output element
+    int64_t size[3];       // W, H, D (unused axes = 1)
+    int64_t cell_off;      // first cell of the field in the batch's cell space
+    int64_t xline_off;     // first x-line of the field in the batch's x-line space
+    double x0[3];
+    double step[3];
+    double sigma[3];
+    int32_t T[3];          // half kernel size per axis
+    int32_t pad1;
+    double alpha[3];       // tail value per axis (0 for plain convolution)
+    double pscale[3];      // (1/(K+2*alpha))^num_iter per axis, FP32C normalisation
+    double thr;            // reference NaN-mask threshold (FP64 path)
+    double thr_fast;       // thr * prod(pscale) (FP32C path)
+};
+
+// One workgroup of a fast line pass: `nlines` lines of one field whose first
+// element offsets (within the field) are base + l*line_stride.
+struct PassWork {
+    int32_t field;
+    int32_t nlines;
+    int64_t base;
+};
+
+// Per-field geometry of one fast line pass along one axis.
+struct PassField {
+    int64_t len;           // line length L
+    int64_t line_stride;   // offset between consecutive lines of a workgroup
+    int64_t elem_stride;   // offset between consecutive elements of a line
+    int32_t T;
+    int32_t lpl;           // lanes per line (power of two, <= 64)
+    double alpha;
+    double pscale;         // applied to the pass result
+};
+
+// Contribution record of the sorted injection list.
+struct Contrib {
+    uint32_t key;          // global cell index (kInvalidKey when outside)
+    uint32_t pidx;         // global point index
+};
+
+}  // namespace fb
+
+// ---- host-side error plumbing (shared by the .hip translation units) ------
+namespace fb {
+void set_error(const std::string &msg);
+const char *last_error();
+}  // namespace fb
+
+#define FB_HIP_CHECK(expr)                                                          \
+    do {                                                                            \
+        hipError_t _e = (expr);                                                     \
+        if (_e != hipSuccess) {                                                     \
+            fb::set_error(std::string("HIP error '") + hipGetErrorString(_e) +      \
+                          "' at " __FILE__ ":" + std::to_string(__LINE__) + " in " #expr); \
+            return FB_ERR_HIP;                                                      \
+        }                                                                           \
+    } while (0)

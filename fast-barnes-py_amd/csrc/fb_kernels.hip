@@ -1,0 +1,764 @@
+// fb_kernels.hip -- device kernels of fast Barnes interpolation on gfx950.
+//
+// Pipeline of one batch (every stage covers all fields of the batch):
+//   k_offsets       value offset (min+max)/2 per field           interpolation.py:204-212
+//   k_contrib       per point: bilinear corner cells -> (cell key, point) pairs,
+//                   x-line counts                                 interpolation.py:218-322
+//   (fb_sort.hip)   stable radix sort of the pairs by cell, scan of the line counts
+//   FP64 path:      k_exact_inject, k_exact_lines (per axis), k_exact_finalize
+//                   -- the reference's operation order, one thread per line
+//   FP32C path:     k_fast_pass<inject> along x, k_fast_pass along y (and z);
+//                   quotient, offset and NaN mask fused into the last pass
+#include <cmath>
+#include <mutex>
+#include <string>
+
+#include "fb_launch.h"
+
+namespace fb {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSegMax = 64;   // maximum elements of one line segment held by one lane
+
+__device__ __forceinline__ int find_seg(const int64_t *__restrict__ prefix, int n, int64_t i)
+{
+    // largest f in [0, n) with prefix[f] <= i
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// value offsets: (min + max)/2 over all values of a field, NaN if any value is NaN
+// (numpy's amin/amax propagate NaN), interpolation.py:209
+
+__global__ __launch_bounds__(kBlock) void k_offsets(const FieldDev *__restrict__ fields,
+                                                    const double *__restrict__ val, double *__restrict__ offsets)
+{
+    __shared__ double smin[kBlock], smax[kBlock];
+    __shared__ int snan;
+    const FieldDev &F = fields[blockIdx.x];
+    const int tid = threadIdx.x;
+    if (tid == 0) snan = 0;
+    double mn = INFINITY, mx = -INFINITY;
+    int isn = 0;
+    for (int64_t k = tid; k < F.n; k += kBlock) {
+        const double v = val[F.pt_off + k];
+        if (v != v) isn = 1;
+        mn = fmin(mn, v);
+        mx = fmax(mx, v);
+    }
+    smin[tid] = mn;
+    smax[tid] = mx;
+    __syncthreads();
+    if (isn) snan = 1;
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            smin[tid] = fmin(smin[tid], smin[tid + s]);
+            smax[tid] = fmax(smax[tid], smax[tid + s]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) offsets[blockIdx.x] = snan ? __builtin_nan("") : (smin[0] + smax[0]) / 2.0;
+}
+
+// ---------------------------------------------------------------------------
+// grid position of a point: xc = (p - x0)/step per axis, dropped when outside
+// [0, size-1) on any axis (interpolation.py:247-250).  NaN coordinates are
+// dropped as well (the reference's int(NaN) is undefined).
+
+struct PointPos {
+    int64_t i[3];
+    double w[3];
+};
+
+__device__ __forceinline__ bool point_pos(const FieldDev &F, const double *__restrict__ p, PointPos &pp)
+{
+    bool inside = true;
+    for (int m = 0; m < 3; ++m) {
+        pp.i[m] = 0;
+        pp.w[m] = 0.0;
+    }
+    for (int m = 0; m < F.dim; ++m) {
+        const double c = (p[m] - F.x0[m]) / F.step[m];
+        if (!(c >= 0.0) || c >= (double)(F.size[m] - 1)) inside = false;
+        const int64_t ci = inside ? (int64_t)c : 0;
+        pp.i[m] = ci;
+        pp.w[m] = c - (double)ci;
+    }
+    return inside;
+}
+
+// bilinear / trilinear corner weight of corner (bx, by, bz), formed left to right as
+// in the reference, e.g. (1.0-xw)*(1.0-yw) at interpolation.py:256
+__device__ __forceinline__ double corner_weight(int dim, const PointPos &pp, int64_t bx, int64_t by, int64_t bz)
+{
+    double w = bx ? pp.w[0] : (1.0 - pp.w[0]);
+    if (dim >= 2) w = w * (by ? pp.w[1] : (1.0 - pp.w[1]));
+    if (dim >= 3) w = w * (bz ? pp.w[2] : (1.0 - pp.w[2]));
+    return w;
+}
+
+__global__ __launch_bounds__(kBlock) void k_contrib(const FieldDev *__restrict__ fields,
+                                                    const int64_t *__restrict__ pt_prefix, int nfields,
+                                                    int64_t npoints, int corners, const double *__restrict__ pts,
+                                                    uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                    uint32_t *__restrict__ line_count)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= npoints) return;
+    const int f = find_seg(pt_prefix, nfields, i);
+    const FieldDev &F = fields[f];
+    const int dim = F.dim;
+    PointPos pp;
+    const bool inside = point_pos(F, pts + i * dim, pp);
+    const int64_t slot = i * corners;
+    if (!inside) {
+        for (int cb = 0; cb < corners; ++cb) {
+            keys[slot + cb] = kInvalidKey;
+            vals[slot + cb] = (uint32_t)i;
+        }
+        return;
+    }
+    const int64_t W = F.size[0], H = F.size[1];
+    for (int cb = 0; cb < corners; ++cb) {
+        const int64_t bx = cb & 1, by = (cb >> 1) & 1, bz = (cb >> 2) & 1;
+        const int64_t cell = ((pp.i[2] + bz) * H + (pp.i[1] + by)) * W + (pp.i[0] + bx);
+        keys[slot + cb] = (uint32_t)(F.cell_off + cell);
+        vals[slot + cb] = (uint32_t)i;
+    }
+    for (int q = 0; q < corners / 2; ++q) {
+        const int64_t by = q & 1, bz = q >> 1;
+        const int64_t line = (pp.i[2] + bz) * H + (pp.i[1] + by);
+        atomicAdd(&line_count[F.xline_off + line], 2u);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 path, reference operation order
+
+// One thread per x-line: walks the line's contributions (sorted by cell, point
+// order inside a cell) and adds w*val_c and w to the FP64 grids in that order,
+// which is the per-cell order of the reference's point loop.
+__global__ __launch_bounds__(kBlock) void k_exact_inject(
+    const FieldDev *__restrict__ fields, const int64_t *__restrict__ xline_prefix, int nfields, int64_t nxlines,
+    const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals, const uint32_t *__restrict__ line_start,
+    const uint32_t *__restrict__ line_count, const double *__restrict__ pts, const double *__restrict__ val,
+    const double *__restrict__ offsets, double *__restrict__ vg, double *__restrict__ wg)
+{
+    const int64_t gl = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (gl >= nxlines) return;
+    const int f = find_seg(xline_prefix, nfields, gl);
+    const FieldDev &F = fields[f];
+    const int64_t line = gl - F.xline_off;
+    const int64_t W = F.size[0], H = F.size[1];
+    const int64_t y = line % H, z = line / H;
+    const uint32_t s = line_start[gl], c = line_count[gl];
+    const double off = offsets[f];
+    for (uint32_t e = s; e < s + c; ++e) {
+        const uint32_t key = keys[e];
+        const uint32_t p = vals[e];
+        const int64_t cell = (int64_t)key - F.cell_off;
+        const int64_t x = cell - line * W;
+        if ((uint64_t)x >= (uint64_t)W) continue;   // cannot happen for a consistent sort
+        PointPos pp;
+        point_pos(F, pts + (int64_t)p * F.dim, pp);
+        const double w = corner_weight(F.dim, pp, x - pp.i[0], y - pp.i[1], z - pp.i[2]);
+        const double vc = val[p] - off;
+        vg[F.cell_off + cell] += w * vc;
+        wg[F.cell_off + cell] += w;
+    }
+}
+
+// _accumulate_tail_array (interpolation.py:484-533) and _accumulate_array (:729-772)
+// on one strided line, ping-ponging between the grid line G and a scratch line S.
+__device__ void exact_line(double *__restrict__ G, double *__restrict__ S, int64_t L, int64_t st, int64_t rect_len,
+                           int num_iter, int tail, double alpha)
+{
+    const int64_t h0 = (rect_len - 1) / 2;
+    const int64_t h0_1 = h0 + 1;
+    const int64_t h1 = rect_len - h0;
+    double *in = G;
+    double *h = S;
+#define IN(i) in[(i) * st]
+#define H_(i) h[(i) * st]
+    for (int it = 0; it < num_iter; ++it) {
+        double accu = 0.0;
+        int64_t k;
+        if (tail) {
+            for (k = -h0; k < 0; ++k) accu += IN(k + h0);
+            for (k = 0; k < h1; ++k) {
+                accu += IN(k + h0);
+                H_(k) = accu + alpha * IN(k + h0_1);
+            }
+            for (k = h1; k < L - h0_1; ++k) {
+                accu += (IN(k + h0) - IN(k - h1));
+                H_(k) = accu + alpha * (IN(k - h1) + IN(k + h0_1));
+            }
+            k = L - h0_1;
+            accu += (IN(k + h0) - IN(k - h1));
+            H_(k) = accu + alpha * IN(k - h1);
+            for (k = L - h0; k < L; ++k) {
+                accu -= IN(k - h1);
+                H_(k) = accu + alpha * IN(k - h1);
+            }
+        } else {
+            for (k = -h0; k < 0; ++k) accu += IN(k + h0);
+            for (k = 0; k < h1; ++k) {
+                accu += IN(k + h0);
+                H_(k) = accu;
+            }
+            for (k = h1; k < L - h0; ++k) {
+                accu += (IN(k + h0) - IN(k - h1));
+                H_(k) = accu;
+            }
+            for (k = L - h0; k < L; ++k) {
+                accu -= IN(k - h1);
+                H_(k) = accu;
+            }
+        }
+        double *t = in; in = h; h = t;
+    }
+    if (in != G)
+        for (int64_t i = 0; i < L; ++i) G[i * st] = in[i * st];
+#undef IN
+#undef H_
+}
+
+__global__ __launch_bounds__(kBlock) void k_exact_lines(const FieldDev *__restrict__ fields,
+                                                        const int64_t *__restrict__ line_prefix, int nfields,
+                                                        int64_t nslots, int axis, int rect_override,
+                                                        double *__restrict__ vg, double *__restrict__ wg,
+                                                        double *__restrict__ vs, double *__restrict__ ws)
+{
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= nslots) return;
+    const int f = find_seg(line_prefix, nfields, g);
+    const FieldDev &F = fields[f];
+    const int64_t r = g - line_prefix[f];
+    const int which = (int)(r & 1);
+    const int64_t l = r >> 1;
+    const int64_t W = F.size[0], H = F.size[1], D = F.size[2];
+    int64_t L, st, base;
+    if (axis == 0) {
+        L = W; st = 1; base = l * W;
+    } else if (axis == 1) {
+        L = H; st = W; base = (l / W) * W * H + (l % W);
+    } else {
+        L = D; st = W * H; base = l;
+    }
+    const int64_t rect = rect_override > 0 ? rect_override : 2 * (int64_t)F.T[axis] + 1;
+    double *G = (which ? wg : vg) + F.cell_off + base;
+    double *S = (which ? ws : vs) + F.cell_off + base;
+    exact_line(G, S, L, st, rect, F.num_iter, F.tail, F.alpha[axis]);
+}
+
+// wg[wg < thr] = NaN, then (vg/wg + offset).astype(float32): interpolation.py:428-430, :367
+__global__ __launch_bounds__(kBlock) void k_exact_finalize(const FieldDev *__restrict__ fields,
+                                                           const int64_t *__restrict__ cell_prefix, int nfields,
+                                                           int64_t ncells, const double *__restrict__ vg,
+                                                           const double *__restrict__ wg,
+                                                           const double *__restrict__ offsets, float *__restrict__ out)
+{
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= ncells) return;
+    const int f = find_seg(cell_prefix, nfields, g);
+    const FieldDev &F = fields[f];
+    const int64_t i = g - F.cell_off;
+    const double w = wg[g];
+    const double v = vg[g];
+    out[F.out_off + i] = (w < F.thr) ? __builtin_nanf("") : (float)(v / w + offsets[f]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_mask(const FieldDev *__restrict__ fields,
+                                                 const int64_t *__restrict__ cell_prefix, int nfields, int64_t ncells,
+                                                 double *__restrict__ wg, int fast)
+{
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= ncells) return;
+    const FieldDev &F = fields[find_seg(cell_prefix, nfields, g)];
+    const double thr = fast ? F.thr_fast : F.thr;
+    if (wg[g] < thr) wg[g] = __builtin_nan("");
+}
+
+// ---------------------------------------------------------------------------
+// FP32C path
+//
+// One workgroup processes `nlines` lines of one field held in LDS as float32,
+// with T+1 zeros of padding on both sides of every line (the reference's zero
+// padding, interpolation.py:503-523).  A line is split into lpl contiguous
+// segments, one per lane; each lane re-anchors its window sum at its segment
+// start with a direct (2T+1)-term sum in FP64 and slides it across the segment
+// in FP64, so no FP32 running sum ever spans more than one segment.  Every
+// iteration is computed into registers, then written back in place after a
+// barrier.  The last iteration multiplies by (1/(K+2*alpha))^n and, on the last
+// pass, forms the masked FP64 quotient directly from the FP64 window sums.
+
+template <bool FINAL>
+__device__ __forceinline__ void slide(const float *__restrict__ lv, const float *__restrict__ lw, int s, int n, int T,
+                                      double alpha, double scale, double thr, double off, float (&ra)[kSegMax],
+                                      float (&rb)[kSegMax])
+{
+    double av = 0.0, aw = 0.0;
+#pragma unroll 4
+    for (int i = s - T; i <= s + T; ++i) {
+        av += (double)lv[i];
+        aw += (double)lw[i];
+    }
+    double pv = (double)lv[s - T - 1];
+    double pw = (double)lw[s - T - 1];
+#pragma unroll
+    for (int j = 0; j < kSegMax; ++j) {
+        if (j < n) {
+            const int k = s + j;
+            const double addv = (double)lv[k + T + 1];
+            const double subv = (double)lv[k - T];
+            const double addw = (double)lw[k + T + 1];
+            const double subw = (double)lw[k - T];
+            const double ov = fma(alpha, pv + addv, av) * scale;
+            const double ow = fma(alpha, pw + addw, aw) * scale;
+            av += addv - subv;
+            aw += addw - subw;
+            pv = subv;
+            pw = subw;
+            if (FINAL) {
+                ra[j] = (ow < thr) ? __builtin_nanf("") : (float)(ov / ow + off);
+            } else {
+                ra[j] = (float)ov;
+                rb[j] = (float)ow;
+            }
+        }
+    }
+}
+
+template <bool INJECT>
+__global__ __launch_bounds__(kBlock) void k_fast_pass(
+    const FieldDev *__restrict__ fields, const double *__restrict__ offsets, const PassWork *__restrict__ work,
+    const PassField *__restrict__ passes, const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
+    const uint32_t *__restrict__ line_start, const uint32_t *__restrict__ line_count, const double *__restrict__ pts,
+    const double *__restrict__ val, float *__restrict__ gv, float *__restrict__ gw, float *__restrict__ out,
+    int final_pass)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const PassWork wk = work[blockIdx.x];
+    const FieldDev &F = fields[wk.field];
+    const PassField P = passes[wk.field];
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int L = (int)P.len;
+    const int T = P.T;
+    const int lpl = P.lpl;
+    const int lines_cap = nthr / lpl;
+    const int pad = T + 1;
+    const int lstride = L + 2 * pad;
+    const int nl = wk.nlines;
+    float *sv = reinterpret_cast<float *>(smem);
+    float *sw = sv + (size_t)lines_cap * lstride;
+    const int my_line = tid / lpl;
+    const int sid = tid - my_line * lpl;
+    const bool line_ok = my_line < nl;
+    const double off = offsets[wk.field];
+
+    const int tot = lines_cap * lstride;
+    for (int i = tid; i < tot; i += nthr) {
+        sv[i] = 0.f;
+        sw[i] = 0.f;
+    }
+    __syncthreads();
+
+    if (!INJECT) {
+        const float *vin = gv + F.cell_off + wk.base;
+        const float *win = gw + F.cell_off + wk.base;
+        const int cnt = nl * L;
+        if (P.line_stride == 1) {
+            for (int idx = tid; idx < cnt; idx += nthr) {
+                const int l = idx % nl, j = idx / nl;
+                const int64_t g = l + (int64_t)j * P.elem_stride;
+                sv[l * lstride + pad + j] = vin[g];
+                sw[l * lstride + pad + j] = win[g];
+            }
+        } else {
+            for (int idx = tid; idx < cnt; idx += nthr) {
+                const int j = idx % L, l = idx / L;
+                const int64_t g = (int64_t)l * P.line_stride + (int64_t)j * P.elem_stride;
+                sv[l * lstride + pad + j] = vin[g];
+                sw[l * lstride + pad + j] = win[g];
+            }
+        }
+    } else {
+        // Deterministic injection.  The line's contributions arrive sorted by cell (point
+        // order inside a cell).  The first wave of the line takes them in chunks of up to
+        // 64; a segmented suffix sum over lanes of equal cell (fixed tree order) gives each
+        // run's total to its first lane, which adds it to the cell.  Chunks of one line are
+        // handled by one wave in program order, so the result does not depend on timing.
+        const int iw = lpl < 64 ? lpl : 64;
+        const bool injector = line_ok && sid < iw;
+        const int64_t W = F.size[0], H = F.size[1];
+        const int64_t line_in_field = wk.base / W + my_line;
+        uint32_t ls = 0, lc = 0;
+        if (injector) {
+            const int64_t gl = F.xline_off + line_in_field;
+            ls = line_start[gl];
+            lc = line_count[gl];
+        }
+        int nch = injector ? (int)((lc + iw - 1) / iw) : 0;
+        for (int d = 32; d >= 1; d >>= 1) nch = max(nch, __shfl_xor(nch, d));
+        const int64_t ly = line_in_field % H, lz = line_in_field / H;
+        for (int ch = 0; ch < nch; ++ch) {
+            const uint32_t e = (uint32_t)ch * iw + sid;
+            const bool valid = injector && e < lc;
+            int x = -1;
+            double cv = 0.0, cw = 0.0;
+            if (valid) {
+                const uint32_t key = keys[ls + e];
+                const uint32_t p = vals[ls + e];
+                const int64_t cell = (int64_t)key - F.cell_off;
+                const int64_t xl = cell - line_in_field * W;
+                x = ((uint64_t)xl < (uint64_t)W) ? (int)xl : -1;   // -1 cannot happen for a consistent sort
+                PointPos pp;
+                point_pos(F, pts + (int64_t)p * F.dim, pp);
+                const double w = corner_weight(F.dim, pp, x - pp.i[0], ly - pp.i[1], lz - pp.i[2]);
+                cv = w * (val[p] - off);
+                cw = w;
+            }
+            for (int d = 1; d < iw; d <<= 1) {
+                const double ov = __shfl_down(cv, d, iw);
+                const double ow = __shfl_down(cw, d, iw);
+                const int ox = __shfl_down(x, d, iw);
+                if (sid + d < iw && ox == x) {
+                    cv += ov;
+                    cw += ow;
+                }
+            }
+            const int px = __shfl_up(x, 1, iw);
+            if (valid && x >= 0 && (sid == 0 || px != x)) {
+                float *pv = sv + my_line * lstride + pad + x;
+                float *pw = sw + my_line * lstride + pad + x;
+                *pv = (float)((double)*pv + cv);
+                *pw = (float)((double)*pw + cw);
+            }
+        }
+    }
+    __syncthreads();
+
+    // n-fold tailed box filter, in place
+    int seg = (L + lpl - 1) / lpl;
+    if (seg < kSegMax) seg |= 1;   // odd segment stride: fewer LDS bank conflicts
+    const int s0 = sid * seg;
+    int myseg = L - s0;
+    myseg = myseg < 0 ? 0 : (myseg > seg ? seg : myseg);
+    float *lv = sv + (size_t)(line_ok ? my_line : 0) * lstride + pad;
+    float *lw = sw + (size_t)(line_ok ? my_line : 0) * lstride + pad;
+    const double alpha = P.alpha;
+    const int niter = F.num_iter;
+    float ra[kSegMax], rb[kSegMax];
+    for (int it = 0; it < niter; ++it) {
+        const bool last = it == niter - 1;
+        const bool fin = last && final_pass;
+        const double scale = last ? P.pscale : 1.0;
+        if (line_ok && myseg > 0) {
+            if (fin) slide<true>(lv, lw, s0, myseg, T, alpha, scale, F.thr_fast, off, ra, rb);
+            else slide<false>(lv, lw, s0, myseg, T, alpha, scale, 0.0, 0.0, ra, rb);
+        }
+        __syncthreads();
+        if (line_ok) {
+#pragma unroll
+            for (int j = 0; j < kSegMax; ++j) {
+                if (j < myseg) {
+                    lv[s0 + j] = ra[j];
+                    if (!fin) lw[s0 + j] = rb[j];
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // store (coalesced along the unit-stride dimension)
+    const int cnt = nl * L;
+    if (final_pass) {
+        float *o = out + F.out_off + wk.base;
+        if (P.line_stride == 1) {
+            for (int idx = tid; idx < cnt; idx += nthr) {
+                const int l = idx % nl, j = idx / nl;
+                o[l + (int64_t)j * P.elem_stride] = sv[l * lstride + pad + j];
+            }
+        } else {
+            for (int idx = tid; idx < cnt; idx += nthr) {
+                const int j = idx % L, l = idx / L;
+                o[(int64_t)l * P.line_stride + (int64_t)j * P.elem_stride] = sv[l * lstride + pad + j];
+            }
+        }
+    } else {
+        float *vo = gv + F.cell_off + wk.base;
+        float *wo = gw + F.cell_off + wk.base;
+        if (P.line_stride == 1) {
+            for (int idx = tid; idx < cnt; idx += nthr) {
+                const int l = idx % nl, j = idx / nl;
+                const int64_t g = l + (int64_t)j * P.elem_stride;
+                vo[g] = sv[l * lstride + pad + j];
+                wo[g] = sw[l * lstride + pad + j];
+            }
+        } else {
+            for (int idx = tid; idx < cnt; idx += nthr) {
+                const int j = idx % L, l = idx / L;
+                const int64_t g = (int64_t)l * P.line_stride + (int64_t)j * P.elem_stride;
+                vo[g] = sv[l * lstride + pad + j];
+                wo[g] = sw[l * lstride + pad + j];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// elementwise helpers
+
+// zero fill by kernel (a fill kernel instead of hipMemsetAsync keeps the batch graph free
+// of memset nodes)
+__global__ __launch_bounds__(kBlock) void k_zero_u32(uint32_t *__restrict__ p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_f64_to_f32(const double *__restrict__ in, float *__restrict__ out,
+                                                       int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_f32_to_f64(const float *__restrict__ in, double *__restrict__ out,
+                                                       int64_t n, double scale)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = (double)in[i] * scale;
+}
+
+// ---------------------------------------------------------------------------
+// naive (interpolation.py:861-938) and radius (:809-855, brute force in place of the
+// kd-tree, same inclusion test sqr_dist <= radius^2) in FP64, one thread per node.
+
+__global__ __launch_bounds__(kBlock) void k_naive(const FieldDev *__restrict__ fp, int radius, double r2, double mdw,
+                                                  const double *__restrict__ pts, const double *__restrict__ val,
+                                                  const double *__restrict__ offsets, double *__restrict__ out,
+                                                  int64_t ncells)
+{
+    __shared__ double sp[3][kBlock];
+    __shared__ double sval[kBlock];
+    const FieldDev F = *fp;
+    const int tid = threadIdx.x;
+    const int64_t g = (int64_t)blockIdx.x * kBlock + tid;
+    const int dim = F.dim;
+    const int64_t W = F.size[0], H = F.size[1];
+    const int64_t x = g % W, y = (g / W) % H, z = g / (W * H);
+    const double c0 = F.x0[0] + (double)x * F.step[0];
+    const double c1 = dim > 1 ? F.x0[1] + (double)y * F.step[1] : 0.0;
+    const double c2 = dim > 2 ? F.x0[2] + (double)z * F.step[2] : 0.0;
+    // scale = 2*sigma**2 per axis (interpolation.py:872)
+    const double s0 = 2 * (F.sigma[0] * F.sigma[0]);
+    const double s1 = 2 * (F.sigma[1] * F.sigma[1]);
+    const double s2 = 2 * (F.sigma[2] * F.sigma[2]);
+    const double off = offsets[0];
+    double ws = 0.0, wt = 0.0;
+    for (int64_t b = 0; b < F.n; b += kBlock) {
+        __syncthreads();
+        const int64_t k = b + tid;
+        if (k < F.n) {
+            for (int m = 0; m < dim; ++m) sp[m][tid] = pts[k * dim + m];
+            sval[tid] = val[k] - off;
+        }
+        __syncthreads();
+        const int cnt = (int)min((int64_t)kBlock, F.n - b);
+        if (g < ncells) {
+            if (!radius) {
+                for (int t = 0; t < cnt; ++t) {
+                    const double d0 = sp[0][t] - c0;
+                    double d = d0 * d0 / s0;
+                    if (dim > 1) {
+                        const double d1 = sp[1][t] - c1;
+                        d = d + d1 * d1 / s1;
+                    }
+                    if (dim > 2) {
+                        const double d2 = sp[2][t] - c2;
+                        d = d + d2 * d2 / s2;
+                    }
+                    const double w = exp(-d);
+                    ws += w * sval[t];
+                    wt += w;
+                }
+            } else {
+                for (int t = 0; t < cnt; ++t) {
+                    const double dx = c0 - sp[0][t];
+                    const double dy = c1 - sp[1][t];
+                    const double d2 = dx * dx + dy * dy;
+                    if (d2 <= r2) {
+                        const double w = exp(-d2 / s0);
+                        ws += w * sval[t];
+                        wt += w;
+                    }
+                }
+            }
+        }
+    }
+    if (g < ncells) {
+        if (!radius) out[g] = wt > 0.0 ? ws / wt + off : __builtin_nan("");
+        else out[g] = wt >= mdw ? ws / wt + off : __builtin_nan("");
+    }
+}
+
+inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+
+int launch_offsets(const FieldDev *d_fields, int nfields, const double *d_val, double *d_offsets, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_offsets, dim3(nfields), dim3(kBlock), 0, s, d_fields, d_val, d_offsets);
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
+int launch_contrib(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints, int corners,
+                   const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_line_count, hipStream_t s)
+{
+    if (npoints <= 0) return FB_OK;
+    hipLaunchKernelGGL(k_contrib, dim3(nblocks(npoints)), dim3(kBlock), 0, s, d_fields, d_pt_prefix, nfields,
+                       npoints, corners, d_pts, d_keys, d_vals, d_line_count);
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
+int launch_exact_inject(const FieldDev *d_fields, const int64_t *d_xline_prefix, int nfields, int64_t nxlines,
+                        const uint32_t *d_keys, const uint32_t *d_vals, const uint32_t *d_line_start,
+                        const uint32_t *d_line_count, const double *d_pts, const double *d_val,
+                        const double *d_offsets, double *d_vg, double *d_wg, hipStream_t s)
+{
+    if (nxlines <= 0) return FB_OK;
+    hipLaunchKernelGGL(k_exact_inject, dim3(nblocks(nxlines)), dim3(kBlock), 0, s, d_fields, d_xline_prefix,
+                       nfields, nxlines, d_keys, d_vals, d_line_start, d_line_count, d_pts, d_val, d_offsets, d_vg,
+                       d_wg);
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
+int launch_exact_lines(const FieldDev *d_fields, const int64_t *d_line_prefix, int nfields, int64_t nslots,
+                       int axis, int rect_len, double *d_vg, double *d_wg, double *d_vs, double *d_ws, hipStream_t s)
+{
+    if (nslots <= 0) return FB_OK;
+    hipLaunchKernelGGL(k_exact_lines, dim3(nblocks(nslots)), dim3(kBlock), 0, s, d_fields, d_line_prefix, nfields,
+                       nslots, axis, rect_len, d_vg, d_wg, d_vs, d_ws);
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
+int launch_exact_finalize(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfields, int64_t ncells,
+                          const double *d_vg, const double *d_wg, const double *d_offsets, float *d_out,
+                          hipStream_t s)
+{
+    if (ncells <= 0) return FB_OK;
+    hipLaunchKernelGGL(k_exact_finalize, dim3(nblocks(ncells)), dim3(kBlock), 0, s, d_fields, d_cell_prefix,
+                       nfields, ncells, d_vg, d_wg, d_offsets, d_out);
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
+int launch_mask(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfields, int64_t ncells, double *d_wg,
+                int fast, hipStream_t s)
+{
+    if (ncells <= 0) return FB_OK;
+    hipLaunchKernelGGL(k_mask, dim3(nblocks(ncells)), dim3(kBlock), 0, s, d_fields, d_cell_prefix, nfields, ncells,
+                       d_wg, fast);
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
+int fast_pass_max_lds() { return 160 * 1024; }
+
+// Raises the dynamic-LDS limit of the fast passes once per process (before any
+// stream capture, where attribute calls are not allowed).
+int fast_pass_init()
+{
+    static std::once_flag once;
+    static hipError_t err = hipSuccess;
+    std::call_once(once, [] {
+        err = hipFuncSetAttribute((const void *)k_fast_pass<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  fast_pass_max_lds());
+        if (err == hipSuccess)
+            err = hipFuncSetAttribute((const void *)k_fast_pass<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      fast_pass_max_lds());
+    });
+    if (err != hipSuccess) {
+        (void)hipGetLastError();
+        set_error(std::string("hipFuncSetAttribute failed: ") + hipGetErrorString(err));
+        return FB_ERR_HIP;
+    }
+    return FB_OK;
+}
+
+int launch_fast_pass(bool inject, const FieldDev *d_fields, const double *d_offsets, const PassWork *d_work,
+                     int nwork, const PassField *d_pass, int threads, size_t lds_bytes, const uint32_t *d_keys,
+                     const uint32_t *d_vals, const uint32_t *d_line_start, const uint32_t *d_line_count,
+                     const double *d_pts, const double *d_val, float *d_v, float *d_w, float *d_out, int final_pass,
+                     hipStream_t s)
+{
+    if (nwork <= 0) return FB_OK;
+    if (threads != kBlock) {
+        set_error("fast pass: unsupported workgroup size");
+        return FB_ERR_UNSUPPORTED;
+    }
+    if (inject) {
+        hipLaunchKernelGGL(k_fast_pass<true>, dim3(nwork), dim3(threads), lds_bytes, s, d_fields, d_offsets, d_work,
+                           d_pass, d_keys, d_vals, d_line_start, d_line_count, d_pts, d_val, d_v, d_w, d_out,
+                           final_pass);
+    } else {
+        hipLaunchKernelGGL(k_fast_pass<false>, dim3(nwork), dim3(threads), lds_bytes, s, d_fields, d_offsets,
+                           d_work, d_pass, d_keys, d_vals, d_line_start, d_line_count, d_pts, d_val, d_v, d_w, d_out,
+                           final_pass);
+    }
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
+int launch_zero(void *p, int64_t bytes, hipStream_t s)
+{
+    const int64_t n = bytes / 4;
+    if (n <= 0) return FB_OK;
+    hipLaunchKernelGGL(k_zero_u32, dim3(nblocks(n)), dim3(kBlock), 0, s, (uint32_t *)p, n);
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
+int launch_f64_to_f32(const double *in, float *out, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return FB_OK;
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(nblocks(n)), dim3(kBlock), 0, s, in, out, n);
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
+int launch_f32_to_f64(const float *in, double *out, int64_t n, double scale, hipStream_t s)
+{
+    if (n <= 0) return FB_OK;
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(nblocks(n)), dim3(kBlock), 0, s, in, out, n, scale);
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
+int launch_naive(const FieldDev *d_field, int radius, double radius_sqr, double max_dist_weight, const double *d_pts,
+                 const double *d_val, const double *d_offsets, double *d_out, int64_t ncells, hipStream_t s)
+{
+    if (ncells <= 0) return FB_OK;
+    hipLaunchKernelGGL(k_naive, dim3(nblocks(ncells)), dim3(kBlock), 0, s, d_field, radius, radius_sqr,
+                       max_dist_weight, d_pts, d_val, d_offsets, d_out, ncells);
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
+}  // namespace fb

@@ -1,0 +1,60 @@
+// fb_launch.h -- host-side launchers shared between translation units.
+#pragma once
+
+#include "fastbarnes.h"
+#include "fb_internal.h"
+
+namespace fb {
+
+// fb_sort.hip
+int64_t scan_temp_elems(int64_t n);
+int scan_exclusive(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tmp, hipStream_t stream);
+int64_t radix_hist_elems(int64_t n);
+int radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, int64_t n, int bits,
+                     uint32_t *hist, uint32_t *scan_tmp, hipStream_t stream, uint32_t **keys_out,
+                     uint32_t **vals_out);
+
+// fb_kernels.hip -- injection front end
+int launch_offsets(const FieldDev *d_fields, int nfields, const double *d_val, double *d_offsets,
+                   hipStream_t stream);
+int launch_contrib(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
+                   int corners, const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_line_count,
+                   hipStream_t stream);
+
+// fb_kernels.hip -- FP64 path with the reference's operation order
+int launch_exact_inject(const FieldDev *d_fields, const int64_t *d_xline_prefix, int nfields, int64_t nxlines,
+                        const uint32_t *d_keys, const uint32_t *d_vals, const uint32_t *d_line_start,
+                        const uint32_t *d_line_count, const double *d_pts, const double *d_val,
+                        const double *d_offsets, double *d_vg, double *d_wg, hipStream_t stream);
+// One thread per (line, field component); lines of axis `axis` of every field.
+// line_prefix[f] = first (2*line) slot of field f; total = line_prefix[nfields].
+// rect_len < 0 selects 2*T[axis]+1 from the field.
+int launch_exact_lines(const FieldDev *d_fields, const int64_t *d_line_prefix, int nfields, int64_t nslots,
+                       int axis, int rect_len, double *d_vg, double *d_wg, double *d_vs, double *d_ws,
+                       hipStream_t stream);
+int launch_exact_finalize(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfields, int64_t ncells,
+                          const double *d_vg, const double *d_wg, const double *d_offsets, float *d_out,
+                          hipStream_t stream);
+int launch_mask(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfields, int64_t ncells, double *d_wg,
+                int fast, hipStream_t stream);
+
+// fb_kernels.hip -- FP32C path
+int fast_pass_max_lds();
+int fast_pass_init();
+int launch_fast_pass(bool inject, const FieldDev *d_fields, const double *d_offsets, const PassWork *d_work,
+                     int nwork, const PassField *d_pass, int threads, size_t lds_bytes, const uint32_t *d_keys,
+                     const uint32_t *d_vals, const uint32_t *d_line_start, const uint32_t *d_line_count,
+                     const double *d_pts, const double *d_val, float *d_v, float *d_w, float *d_out,
+                     int final_pass, hipStream_t stream);
+
+// fb_kernels.hip -- elementwise helpers
+int launch_zero(void *p, int64_t bytes, hipStream_t stream);   // bytes: multiple of 4
+int launch_f64_to_f32(const double *in, float *out, int64_t n, hipStream_t stream);
+int launch_f32_to_f64(const float *in, double *out, int64_t n, double scale, hipStream_t stream);
+
+// fb_kernels.hip -- naive / radius (O(N*W*H*D) brute force, FP64)
+int launch_naive(const FieldDev *d_field, int radius, double radius_sqr, double max_dist_weight,
+                 const double *d_pts, const double *d_val, const double *d_offsets, double *d_out, int64_t ncells,
+                 hipStream_t stream);
+
+}  // namespace fb

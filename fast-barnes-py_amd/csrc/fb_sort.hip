@@ -1,0 +1,193 @@
+// fb_sort.hip -- exclusive scan and stable LSD radix sort of (key, value) uint32 pairs.
+//
+// Used to order the injection contributions by grid cell while keeping, inside a
+// cell, the point order of the input (the order in which the reference's
+// _inject_data_*d loop adds them, interpolation.py:246-270).  Histogram counts use
+// integer LDS atomics only, so the result is deterministic.
+#include "fb_internal.h"
+#include "fb_launch.h"
+
+namespace fb {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kItems = 16;
+constexpr int kTile = kThreads * kItems;   // 4096 elements per workgroup
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kWaves = kThreads / 64;
+
+// ---- scan ----------------------------------------------------------------
+
+__global__ __launch_bounds__(kThreads) void k_scan_tiles(const uint32_t *__restrict__ in,
+                                                         uint32_t *__restrict__ out,
+                                                         uint32_t *__restrict__ tile_sums, int64_t n)
+{
+    __shared__ uint32_t s_tot[kThreads];
+    const int tid = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)tid * kItems;
+    uint32_t v[kItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const int64_t idx = base + i;
+        v[i] = idx < n ? in[idx] : 0u;
+        sum += v[i];
+    }
+    s_tot[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < kThreads; off <<= 1) {
+        const uint32_t t = tid >= off ? s_tot[tid - off] : 0u;
+        __syncthreads();
+        s_tot[tid] += t;
+        __syncthreads();
+    }
+    uint32_t run = tid ? s_tot[tid - 1] : 0u;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const int64_t idx = base + i;
+        if (idx < n) out[idx] = run;
+        run += v[i];
+    }
+    if (tid == kThreads - 1 && tile_sums) tile_sums[blockIdx.x] = s_tot[kThreads - 1];
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_add(uint32_t *__restrict__ out,
+                                                       const uint32_t *__restrict__ tile_offs, int64_t n)
+{
+    const int64_t idx = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (idx < n) out[idx] += tile_offs[idx / kTile];
+}
+
+// ---- radix sort ----------------------------------------------------------
+
+__global__ __launch_bounds__(kThreads) void k_radix_hist(const uint32_t *__restrict__ keys, int64_t n, int shift,
+                                                         uint32_t *__restrict__ hist, int ntiles)
+{
+    __shared__ uint32_t h[kRadix];
+    const int tid = threadIdx.x;
+    h[tid] = 0u;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+    for (int r = 0; r < kItems; ++r) {
+        const int64_t idx = base + (int64_t)r * kThreads + tid;
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (kRadix - 1)], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)tid * ntiles + blockIdx.x] = h[tid];
+}
+
+__global__ __launch_bounds__(kThreads) void k_radix_scatter(const uint32_t *__restrict__ kin,
+                                                            const uint32_t *__restrict__ vin,
+                                                            uint32_t *__restrict__ kout,
+                                                            uint32_t *__restrict__ vout, int64_t n, int shift,
+                                                            const uint32_t *__restrict__ offs, int ntiles)
+{
+    __shared__ uint32_t running[kRadix];
+    __shared__ uint32_t wcnt[kWaves][kRadix];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const uint64_t lanes_below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    running[tid] = offs[(int64_t)tid * ntiles + blockIdx.x];
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+    for (int r = 0; r < kItems; ++r) {
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0u;
+        __syncthreads();
+        const int64_t idx = base + (int64_t)r * kThreads + tid;
+        const bool valid = idx < n;
+        const uint32_t key = valid ? kin[idx] : 0u;
+        const uint32_t val = valid ? vin[idx] : 0u;
+        const uint32_t d = (key >> shift) & (kRadix - 1);
+        // lanes of this wave holding the same digit (stable rank = lanes below with the same digit)
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < kRadixBits; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lanes_below);
+        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pre = 0;
+            for (int w = 0; w < wave; ++w) pre += wcnt[w][d];
+            const uint32_t pos = running[d] + pre + rank;
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) add += wcnt[w][tid];
+        running[tid] += add;
+        __syncthreads();
+    }
+}
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+int64_t scan_temp_elems(int64_t n)
+{
+    int64_t total = 0;
+    while (n > kTile) {
+        n = cdiv(n, kTile);
+        total += n;
+    }
+    return total + 1;
+}
+
+int scan_exclusive(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tmp, hipStream_t stream)
+{
+    if (n <= 0) return FB_OK;
+    const int64_t ntiles = cdiv(n, kTile);
+    if (ntiles == 1) {
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kThreads), 0, stream, in, out, (uint32_t *)nullptr, n);
+        FB_HIP_CHECK(hipGetLastError());
+        return FB_OK;
+    }
+    uint32_t *sums = tmp;
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(kThreads), 0, stream, in, out, sums, n);
+    FB_HIP_CHECK(hipGetLastError());
+    int rc = scan_exclusive(sums, sums, ntiles, tmp + ntiles, stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)cdiv(n, kThreads)), dim3(kThreads), 0, stream, out,
+                       (const uint32_t *)sums, n);
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
+int64_t radix_hist_elems(int64_t n) { return (int64_t)kRadix * cdiv(n > 0 ? n : 1, kTile); }
+
+int radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, int64_t n, int bits,
+                     uint32_t *hist, uint32_t *scan_tmp, hipStream_t stream, uint32_t **keys_out,
+                     uint32_t **vals_out)
+{
+    uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
+    if (n > 0) {
+        const int ntiles = (int)cdiv(n, kTile);
+        const int64_t nh = (int64_t)kRadix * ntiles;
+        for (int shift = 0; shift < bits; shift += kRadixBits) {
+            hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kThreads), 0, stream, (const uint32_t *)ki, n,
+                               shift, hist, ntiles);
+            FB_HIP_CHECK(hipGetLastError());
+            int rc = scan_exclusive(hist, hist, nh, scan_tmp, stream);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kThreads), 0, stream, (const uint32_t *)ki,
+                               (const uint32_t *)vi, ko, vo, n, shift, (const uint32_t *)hist, ntiles);
+            FB_HIP_CHECK(hipGetLastError());
+            uint32_t *t = ki; ki = ko; ko = t;
+            t = vi; vi = vo; vo = t;
+        }
+    }
+    *keys_out = ki;
+    *vals_out = vi;
+    return FB_OK;
+}
+
+}  // namespace fb

@@ -1,0 +1,112 @@
+"""
+ctypes binding of libfastbarnes.so (the C ABI declared in include/fastbarnes.h).
+
+The library is built in-tree by ``make -C fast-barnes-py_amd`` (or
+``__graft_entry__.build()``).  There is no fallback: if the shared object is
+missing or fails to load, every entry point raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libfastbarnes.so')
+
+FB_OK = 0
+FB_ERR_INVALID_ARGUMENT = 1
+FB_ERR_HIP = 2
+FB_ERR_UNSUPPORTED = 3
+FB_ERR_OUT_OF_MEMORY = 4
+
+FB_METHOD_OPTIMIZED_CONVOLUTION = 0
+FB_METHOD_CONVOLUTION = 1
+FB_METHOD_NAIVE = 2
+FB_METHOD_RADIUS = 3
+
+FB_PREC_FP64 = 0
+FB_PREC_FP32C = 1
+
+FB_FLAG_DEVICE_PTRS = 1
+
+PRECISIONS = {'fp64': FB_PREC_FP64, 'fp32c': FB_PREC_FP32C}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+
+
+class FbField(ctypes.Structure):
+    """Mirror of struct fb_field."""
+    _fields_ = [
+        ('dim', ctypes.c_int32),
+        ('method', ctypes.c_int32),
+        ('num_iter', ctypes.c_int32),
+        ('reserved', ctypes.c_int32),
+        ('n', ctypes.c_int64),
+        ('pt_offset', ctypes.c_int64),
+        ('out_offset', ctypes.c_int64),
+        ('size', ctypes.c_int64 * 3),
+        ('sigma', ctypes.c_double * 3),
+        ('x0', ctypes.c_double * 3),
+        ('step', ctypes.c_double * 3),
+        ('max_dist_weight', ctypes.c_double),
+    ]
+
+
+# name -> (restype, argtypes); the exported symbols of include/fastbarnes.h
+SIGNATURES = {
+    'fb_version': (ctypes.c_int, []),
+    'fb_last_error': (ctypes.c_char_p, []),
+    'fb_device_count': (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    'fb_half_kernel_size': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _i32p]),
+    'fb_tail_value': (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp]),
+    'fb_sigma_effective': (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp]),
+    'fb_threshold': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _i32p, ctypes.c_int, _dp, ctypes.c_double,
+                                    _dp]),
+    'fb_barnes': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _vp, _vp, _dp, _dp, _dp,
+                                 _i64p, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, ctypes.c_uint, _vp]),
+    'fb_accumulate_lines': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_double, _vp, _vp, ctypes.c_uint, _vp]),
+    'fb_convolve_grid': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i64p, _i32p, ctypes.c_int, _dp,
+                                        ctypes.c_double, _vp, _vp, ctypes.c_uint, _vp]),
+    'fb_plan_create': (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(FbField), ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_uint]),
+    'fb_plan_execute': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    'fb_plan_execute_graph': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    'fb_plan_set_timing': (ctypes.c_int, [_vp, ctypes.c_int]),
+    'fb_plan_stage_times': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_int]),
+    'fb_plan_info': (ctypes.c_int, [_vp, _i64p, _i64p]),
+    'fb_plan_destroy': (ctypes.c_int, [_vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libfastbarnes.so once; raises RuntimeError if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError('libfastbarnes.so is not built (expected at %s); run `make -C fast-barnes-py_amd` '
+                               'or __graft_entry__.build()' % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    """Raises RuntimeError with the library's message for a non-zero status."""
+    if rc != FB_OK:
+        msg = lib().fb_last_error()
+        raise RuntimeError(msg.decode() if msg else 'fastbarnes error %d' % rc)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    check(lib().fb_device_count(ctypes.byref(n)))
+    return n.value

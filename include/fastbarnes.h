@@ -1,0 +1,158 @@
+/*
+ * fastbarnes.h -- C ABI of libfastbarnes.so, the MI355X (gfx950) implementation of
+ * fast (convolutional) Barnes interpolation.
+ *
+ * Every entry point returns an int status (FB_OK == 0) and never throws; the
+ * text of the last error of the calling thread is available from fb_last_error().
+ * Pointers are plain host or device pointers; sizes are int64_t.  Arrays follow
+ * the reference's conventions: `size` is given in (x, y, z) order and grids are
+ * stored row-major with index [y][x] / [z][y][x]; pts is N x dim, row-major.
+ *
+ * Reference interfaces replaced (MeteoSwiss fast-barnes-py 2.0.0,
+ * fastbarnes/interpolation.py) are cited beside each declaration.
+ */
+#ifndef FASTBARNES_H
+#define FASTBARNES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define FB_OK 0
+#define FB_ERR_INVALID_ARGUMENT 1   /* maps to the reference's RuntimeError */
+#define FB_ERR_HIP 2                /* a HIP runtime call failed */
+#define FB_ERR_UNSUPPORTED 3        /* arguments valid for the reference, outside this build's limits */
+#define FB_ERR_OUT_OF_MEMORY 4
+
+/* ---- methods (the `method` string of barnes(), interpolation.py:65-76,169-199) */
+#define FB_METHOD_OPTIMIZED_CONVOLUTION 0
+#define FB_METHOD_CONVOLUTION 1
+#define FB_METHOD_NAIVE 2
+#define FB_METHOD_RADIUS 3
+
+/* ---- arithmetic of the convolution methods ----------------------------- */
+/* FB_PREC_FP64: the reference's own FP64 operation order (injection in point order,
+ *   sequential running sums per line, FP64 quotient); float32 results equal the
+ *   reference's bit for bit.
+ * FP32C: FP32 storage with FP64 accumulation of every window sum, FP64 weight
+ *   normalisation and quotient; deterministic, RMSE <= 1e-5 against FP64. */
+#define FB_PREC_FP64 0
+#define FB_PREC_FP32C 1
+
+/* ---- flags ------------------------------------------------------------- */
+#define FB_FLAG_DEVICE_PTRS 1u      /* pts/val/out (and line buffers) are device pointers */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int fb_version(void);
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char *fb_last_error(void);
+
+/* Number of visible HIP devices (0 when none).  Host-only. */
+int fb_device_count(int *count);
+
+/* ---- kernel parameters (host-only, no GPU needed) ---------------------- */
+
+/* Half kernel size T per axis.
+ *   method FB_METHOD_OPTIMIZED_CONVOLUTION: _get_half_kernel_size_opt  interpolation.py:548-552
+ *   method FB_METHOD_CONVOLUTION:           _get_half_kernel_size      interpolation.py:782-785 */
+int fb_half_kernel_size(int method, int dim, const double *sigma, const double *step, int num_iter,
+                        int32_t *half_kernel_size);
+
+/* Tail value alpha per axis: _get_tail_value  interpolation.py:560-569 */
+int fb_tail_value(int dim, const double *sigma, const double *step, int num_iter, double *tail_value);
+
+/* Effective sigma of the plain rect kernel: _get_sigma_effective  interpolation.py:796-803 */
+int fb_sigma_effective(int dim, const double *sigma, const double *step, int num_iter, double *sigma_eff);
+
+/* NaN-mask threshold conv_scale_factor of the convolution methods:
+ *   tailed interpolation.py:424-425 (1D :389-390, 3D :472-473); plain :669-670 (:634-635, :717-718).
+ * kernel_size and tail_value are per axis (tail_value ignored for FB_METHOD_CONVOLUTION). */
+int fb_threshold(int method, int dim, const double *sigma, const double *step, const int32_t *kernel_size,
+                 int num_iter, const double *tail_value, double max_dist_weight, double *threshold);
+
+/* ---- the drop-in ------------------------------------------------------- */
+
+/* barnes(pts, val, sigma, x0, step, size, method, num_iter, max_dist, min_weight)
+ *   interpolation.py:31-199 (dispatch :169-199) -> _interpolate_opt_convol :328-367,
+ *   _interpolate_convol :575-612, _interpolate_naive :861-938, _interpolate_radius :809-855.
+ * max_dist_weight = exp(-max_dist^2/2) as computed at interpolation.py:167.
+ * out: float32 of W*H*D elements for the convolution methods, float64 for naive/radius.
+ * flags: FB_FLAG_DEVICE_PTRS when pts/val/out live in device memory; stream is a
+ * hipStream_t (NULL = the library's stream).  The call returns after `out` is written. */
+int fb_barnes(int method, int precision, int dim, int64_t n, const double *pts, const double *val,
+              const double *sigma, const double *x0, const double *step, const int64_t *size,
+              int num_iter, double max_dist_weight, double min_weight, void *out, unsigned flags,
+              void *stream);
+
+/* ---- line and grid primitives behind the reference's private helpers ---- */
+
+/* num_iter-fold box filter of `nlines` lines of length len (row-major nlines x len, FP64):
+ *   alpha-tailed  _accumulate_tail_array  interpolation.py:484-533  (tail != 0)
+ *   plain         _accumulate_array       interpolation.py:729-772  (tail == 0)
+ * precision FB_PREC_FP64 reproduces the reference bit for bit; FB_PREC_FP32C runs the
+ * fast kernel (FP32 storage, FP64 accumulators).  in and out may alias. */
+int fb_accumulate_lines(int tail, int precision, int64_t nlines, int64_t len, int rect_len, int num_iter,
+                        double alpha, const double *in, double *out, unsigned flags, void *stream);
+
+/* In-place num_iter-fold convolution of the FP64 fields vg, wg (shape size[::-1]) along every
+ * axis followed by wg[wg < threshold] = NaN:
+ *   _convolve_tail_1d/2d/3d interpolation.py:372-479 (tail != 0),
+ *   _convolve_1d/2d/3d      interpolation.py:617-724 (tail == 0). */
+int fb_convolve_grid(int tail, int precision, int dim, const int64_t *size, const int32_t *kernel_size,
+                     int num_iter, const double *tail_value, double threshold, double *vg, double *wg,
+                     unsigned flags, void *stream);
+
+/* ---- batched fields (no reference counterpart: the reference interpolates one field per call) */
+
+/* One field of a batch.  Its points are rows [pt_offset, pt_offset+n) of the batch's
+ * pts array (row-major, dim columns) and val array; its result occupies
+ * size[0]*size[1]*size[2] float32 elements at out_offset of the batch output. */
+typedef struct fb_field {
+    int32_t dim;            /* 1, 2 or 3 (all fields of a plan share it) */
+    int32_t method;         /* FB_METHOD_OPTIMIZED_CONVOLUTION or FB_METHOD_CONVOLUTION */
+    int32_t num_iter;
+    int32_t reserved;
+    int64_t n;
+    int64_t pt_offset;
+    int64_t out_offset;
+    int64_t size[3];        /* unused axes = 1 */
+    double sigma[3];
+    double x0[3];
+    double step[3];
+    double max_dist_weight;
+} fb_field;
+
+typedef struct fb_plan fb_plan;
+
+/* Validates the fields, derives per-field kernel parameters on the host, uploads the
+ * descriptors and allocates the plan's device workspace on the current device. */
+int fb_plan_create(fb_plan **plan, const fb_field *fields, int nfields, int precision, unsigned flags);
+
+/* Runs the batch on `stream`: pts/val/out are device pointers laid out as described
+ * for fb_field.  Asynchronous with respect to the host. */
+int fb_plan_execute(fb_plan *plan, const double *d_pts, const double *d_val, float *d_out, void *stream);
+
+/* Captures fb_plan_execute for fixed buffers into a hipGraph and replays it; the first
+ * call for a (pts, val, out, stream) tuple captures, later calls replay. */
+int fb_plan_execute_graph(fb_plan *plan, const double *d_pts, const double *d_val, float *d_out, void *stream);
+
+/* Per-stage device time (ms) of the last fb_plan_execute run with timing enabled,
+ * measured with HIP events recorded on the execution stream.  names receives a
+ * comma-separated stage list.  enable != 0 turns event recording on. */
+int fb_plan_set_timing(fb_plan *plan, int enable);
+int fb_plan_stage_times(fb_plan *plan, float *ms, int max_stages, int *nstages, char *names, int names_len);
+
+/* Total points of the batch's output grids, and device workspace bytes held. */
+int fb_plan_info(fb_plan *plan, int64_t *total_points, int64_t *workspace_bytes);
+
+int fb_plan_destroy(fb_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTBARNES_H */
