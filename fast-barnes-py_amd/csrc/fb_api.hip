@@ -23,7 +23,8 @@ const char *last_error() { return g_err.c_str(); }
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kSegMax = 63;              // maximum segment length per lane in the fast passes
+constexpr int kSegMax = 40;              // maximum segment length per lane in the fast passes
+                                         // (odd-rounded to 41 = register staging of the kernel)
 constexpr size_t kLdsBudget = 80 * 1024; // two workgroups per CU
 constexpr size_t kLdsMax = 160 * 1024;
 
@@ -243,7 +244,7 @@ struct fb_plan {
     int device = 0;
     int precision = FB_PREC_FP64;
     int nfields = 0, dim = 0, corners = 0;
-    int64_t npoints = 0, ncells = 0, nxlines = 0, nentries = 0, total_out = 0;
+    int64_t npoints = 0, ncells = 0, nxlines = 0, nentries = 0, total_out = 0, total_cells = 0;
     int key_bits = 8;
     std::vector<FieldDev> fh;
     std::vector<DevBuf> bufs;
@@ -435,7 +436,8 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
         }
         P->npoints += d.n;
         P->nxlines += H * D;
-        P->ncells += W * H * D;
+        P->ncells += (W * H * D + 63) / 64 * 64;   // 256-byte aligned field starts (float4 tiles)
+        P->total_cells += W * H * D;
         P->total_out = std::max(P->total_out, d.out_off + W * H * D);
     }
     pt_prefix.push_back(P->npoints);
@@ -714,7 +716,7 @@ int fb_plan_stage_times(fb_plan *P, float *ms, int max_stages, int *nstages, cha
 int fb_plan_info(fb_plan *P, int64_t *total_points, int64_t *workspace_bytes)
 {
     if (!P) return fail(FB_ERR_INVALID_ARGUMENT, "plan is NULL");
-    if (total_points) *total_points = P->ncells;
+    if (total_points) *total_points = P->total_cells;
     if (workspace_bytes) *workspace_bytes = (int64_t)P->ws_bytes;
     return FB_OK;
 }

@@ -84,12 +84,15 @@ __device__ __forceinline__ bool point_pos(const FieldDev &F, const double *__res
         pp.i[m] = 0;
         pp.w[m] = 0.0;
     }
-    for (int m = 0; m < F.dim; ++m) {
-        const double c = (p[m] - F.x0[m]) / F.step[m];
-        if (!(c >= 0.0) || c >= (double)(F.size[m] - 1)) inside = false;
-        const int64_t ci = inside ? (int64_t)c : 0;
-        pp.i[m] = ci;
-        pp.w[m] = c - (double)ci;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        if (m < F.dim) {
+            const double c = (p[m] - F.x0[m]) / F.step[m];
+            if (!(c >= 0.0) || c >= (double)(F.size[m] - 1)) inside = false;
+            const int64_t ci = inside ? (int64_t)c : 0;
+            pp.i[m] = ci;
+            pp.w[m] = c - (double)ci;
+        }
     }
     return inside;
 }
@@ -270,6 +273,7 @@ __global__ __launch_bounds__(kBlock) void k_exact_finalize(const FieldDev *__res
     const int f = find_seg(cell_prefix, nfields, g);
     const FieldDev &F = fields[f];
     const int64_t i = g - F.cell_off;
+    if (i >= F.size[0] * F.size[1] * F.size[2]) return;   // alignment gap between fields
     const double w = wg[g];
     const double v = vg[g];
     out[F.out_off + i] = (w < F.thr) ? __builtin_nanf("") : (float)(v / w + offsets[f]);
@@ -282,6 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_mask(const FieldDev *__restrict__ fi
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (g >= ncells) return;
     const FieldDev &F = fields[find_seg(cell_prefix, nfields, g)];
+    if (g - F.cell_off >= F.size[0] * F.size[1] * F.size[2]) return;   // alignment gap between fields
     const double thr = fast ? F.thr_fast : F.thr;
     if (wg[g] < thr) wg[g] = __builtin_nan("");
 }
@@ -294,18 +299,41 @@ __global__ __launch_bounds__(kBlock) void k_mask(const FieldDev *__restrict__ fi
 // padding, interpolation.py:503-523).  A line is split into lpl contiguous
 // segments, one per lane; each lane re-anchors its window sum at its segment
 // start with a direct (2T+1)-term sum in FP64 and slides it across the segment
-// in FP64, so no FP32 running sum ever spans more than one segment.  Every
-// iteration is computed into registers, then written back in place after a
-// barrier.  The last iteration multiplies by (1/(K+2*alpha))^n and, on the last
-// pass, forms the masked FP64 quotient directly from the FP64 window sums.
+// in FP64, so no running sum ever spans more than one segment and no FP32 sum
+// is ever formed.  Each iteration is computed into registers (one field at a
+// time) and written back in place after a barrier.  The last iteration scales
+// by (1/(K+2*alpha))^n and, on the last pass, forms the masked FP64 quotient
+// directly from the FP64 window sums of both fields.
 
-template <bool FINAL>
-__device__ __forceinline__ void slide(const float *__restrict__ lv, const float *__restrict__ lw, int s, int n, int T,
-                                      double alpha, double scale, double thr, double off, float (&ra)[kSegMax],
-                                      float (&rb)[kSegMax])
+constexpr int kSegArr = 41;   // register staging per lane: segments of up to 41 elements
+
+// one iteration of the tailed box filter over the lane's segment [s, s+n) of line `lin`
+__device__ __forceinline__ void slide1(const float *__restrict__ lin, int s, int n, int T, double alpha, double scale,
+                                       float (&ra)[kSegArr])
+{
+    double a = 0.0;
+    for (int i = s - T; i <= s + T; ++i) a += (double)lin[i];
+    double p = (double)lin[s - T - 1];
+    const float *addp = lin + s + T + 1;
+    const float *subp = lin + s - T;
+#pragma unroll
+    for (int j = 0; j < kSegArr; ++j) {
+        if (j < n) {
+            const double ad = (double)addp[j];
+            const double sb = (double)subp[j];
+            ra[j] = (float)(fma(alpha, p + ad, a) * scale);
+            a += ad - sb;
+            p = sb;
+        }
+    }
+}
+
+// last iteration of the last pass: both fields, masked quotient + offset
+__device__ __forceinline__ void slide_final(const float *__restrict__ lv, const float *__restrict__ lw, int s, int n,
+                                            int T, double alpha, double scale, double thr, double off,
+                                            float (&ra)[kSegArr])
 {
     double av = 0.0, aw = 0.0;
-#pragma unroll 4
     for (int i = s - T; i <= s + T; ++i) {
         av += (double)lv[i];
         aw += (double)lw[i];
@@ -313,7 +341,7 @@ __device__ __forceinline__ void slide(const float *__restrict__ lv, const float 
     double pv = (double)lv[s - T - 1];
     double pw = (double)lw[s - T - 1];
 #pragma unroll
-    for (int j = 0; j < kSegMax; ++j) {
+    for (int j = 0; j < kSegArr; ++j) {
         if (j < n) {
             const int k = s + j;
             const double addv = (double)lv[k + T + 1];
@@ -326,18 +354,123 @@ __device__ __forceinline__ void slide(const float *__restrict__ lv, const float 
             aw += addw - subw;
             pv = subv;
             pw = subw;
-            if (FINAL) {
-                ra[j] = (ow < thr) ? __builtin_nanf("") : (float)(ov / ow + off);
-            } else {
-                ra[j] = (float)ov;
-                rb[j] = (float)ow;
+            ra[j] = (ow < thr) ? __builtin_nanf("") : (float)(ov / ow + off);
+        }
+    }
+}
+
+__device__ __forceinline__ void stage_store(float *__restrict__ lin, int s, int n, const float (&ra)[kSegArr])
+{
+#pragma unroll
+    for (int j = 0; j < kSegArr; ++j)
+        if (j < n) lin[s + j] = ra[j];
+}
+
+__device__ __forceinline__ bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
+// global tile -> LDS.  Tile: nl lines of length L; element (l, j) at g[l*ls + j*es] goes to
+// lds[l*lstride + j].  Either ls == 1 (columns, rows of nl contiguous floats) or es == 1
+// (rows of L contiguous floats).  Float4 accesses when the layout allows.
+__device__ __forceinline__ void tile_load(const float *__restrict__ g, int64_t ls, int64_t es, int nl, int L,
+                                          float *__restrict__ lds, int lstride)
+{
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    if (ls == 1) {
+        if ((nl & 3) == 0 && (es & 3) == 0 && aligned16(g)) {
+            const int nq = nl >> 2;
+            const int rstep = nthr / nq;
+            const int q = tid % nq;
+            const int r0 = tid / nq;
+            if (r0 < rstep) {
+#pragma unroll 4
+                for (int r = r0; r < L; r += rstep) {
+                    const float4 x = *reinterpret_cast<const float4 *>(g + (int64_t)r * es + 4 * q);
+                    float *d = lds + (4 * q) * lstride + r;
+                    d[0] = x.x;
+                    d[lstride] = x.y;
+                    d[2 * lstride] = x.z;
+                    d[3 * lstride] = x.w;
+                }
             }
+        } else {
+            const int rstep = nthr / nl;
+            const int l = tid % nl;
+            const int r0 = tid / nl;
+            if (r0 < rstep) {
+#pragma unroll 4
+                for (int r = r0; r < L; r += rstep) lds[l * lstride + r] = g[(int64_t)r * es + l];
+            }
+        }
+    } else {
+        if ((L & 3) == 0 && (ls & 3) == 0 && aligned16(g)) {
+            const int Lq = L >> 2;
+            for (int l = 0; l < nl; ++l) {
+#pragma unroll 4
+                for (int q = tid; q < Lq; q += nthr) {
+                    const float4 x = *reinterpret_cast<const float4 *>(g + l * ls + 4 * q);
+                    float *d = lds + l * lstride + 4 * q;
+                    d[0] = x.x;
+                    d[1] = x.y;
+                    d[2] = x.z;
+                    d[3] = x.w;
+                }
+            }
+        } else {
+            for (int l = 0; l < nl; ++l)
+#pragma unroll 4
+                for (int j = tid; j < L; j += nthr) lds[l * lstride + j] = g[l * ls + (int64_t)j * es];
+        }
+    }
+}
+
+// LDS -> global tile (inverse of tile_load)
+__device__ __forceinline__ void tile_store(float *__restrict__ g, int64_t ls, int64_t es, int nl, int L,
+                                           const float *__restrict__ lds, int lstride)
+{
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    if (ls == 1) {
+        if ((nl & 3) == 0 && (es & 3) == 0 && aligned16(g)) {
+            const int nq = nl >> 2;
+            const int rstep = nthr / nq;
+            const int q = tid % nq;
+            const int r0 = tid / nq;
+            if (r0 < rstep) {
+#pragma unroll 4
+                for (int r = r0; r < L; r += rstep) {
+                    const float *d = lds + (4 * q) * lstride + r;
+                    *reinterpret_cast<float4 *>(g + (int64_t)r * es + 4 * q) =
+                        make_float4(d[0], d[lstride], d[2 * lstride], d[3 * lstride]);
+                }
+            }
+        } else {
+            const int rstep = nthr / nl;
+            const int l = tid % nl;
+            const int r0 = tid / nl;
+            if (r0 < rstep) {
+#pragma unroll 4
+                for (int r = r0; r < L; r += rstep) g[(int64_t)r * es + l] = lds[l * lstride + r];
+            }
+        }
+    } else {
+        if ((L & 3) == 0 && (ls & 3) == 0 && aligned16(g)) {
+            const int Lq = L >> 2;
+            for (int l = 0; l < nl; ++l) {
+#pragma unroll 4
+                for (int q = tid; q < Lq; q += nthr) {
+                    const float *d = lds + l * lstride + 4 * q;
+                    *reinterpret_cast<float4 *>(g + l * ls + 4 * q) = make_float4(d[0], d[1], d[2], d[3]);
+                }
+            }
+        } else {
+            for (int l = 0; l < nl; ++l)
+#pragma unroll 4
+                for (int j = tid; j < L; j += nthr) g[l * ls + (int64_t)j * es] = lds[l * lstride + j];
         }
     }
 }
 
 template <bool INJECT>
-__global__ __launch_bounds__(kBlock) void k_fast_pass(
+__global__ __launch_bounds__(kBlock, 2) void k_fast_pass(
     const FieldDev *__restrict__ fields, const double *__restrict__ offsets, const PassWork *__restrict__ work,
     const PassField *__restrict__ passes, const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
     const uint32_t *__restrict__ line_start, const uint32_t *__restrict__ line_count, const double *__restrict__ pts,
@@ -364,48 +497,43 @@ __global__ __launch_bounds__(kBlock) void k_fast_pass(
     const bool line_ok = my_line < nl;
     const double off = offsets[wk.field];
 
-    const int tot = lines_cap * lstride;
-    for (int i = tid; i < tot; i += nthr) {
-        sv[i] = 0.f;
-        sw[i] = 0.f;
+    // injection: fetch this line's contribution range before zeroing the LDS
+    const int iw = lpl < 64 ? lpl : 64;
+    const bool injector = INJECT && line_ok && sid < iw;
+    const int64_t W = F.size[0], H = F.size[1];
+    const int64_t line_in_field = INJECT ? wk.base / W + my_line : 0;
+    uint32_t ls = 0, lc = 0;
+    if (injector) {
+        const int64_t gl = F.xline_off + line_in_field;
+        ls = line_start[gl];
+        lc = line_count[gl];
+    }
+
+    if (INJECT) {
+        // zero the lines and their pads
+        float4 *z = reinterpret_cast<float4 *>(smem);
+        const int nz = (2 * lines_cap * lstride) >> 2;
+        for (int i = tid; i < nz; i += nthr) z[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int i = (nz << 2) + tid; i < 2 * lines_cap * lstride; i += nthr) sv[i] = 0.f;
+    } else {
+        // zero the pads only, then load the lines
+        for (int i = tid; i < lines_cap * 2 * pad; i += nthr) {
+            const int l = i / (2 * pad), r = i - l * 2 * pad;
+            const int o = l * lstride + (r < pad ? r : L + r);
+            sv[o] = 0.f;
+            sw[o] = 0.f;
+        }
+        tile_load(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
+        tile_load(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
     }
     __syncthreads();
 
-    if (!INJECT) {
-        const float *vin = gv + F.cell_off + wk.base;
-        const float *win = gw + F.cell_off + wk.base;
-        const int cnt = nl * L;
-        if (P.line_stride == 1) {
-            for (int idx = tid; idx < cnt; idx += nthr) {
-                const int l = idx % nl, j = idx / nl;
-                const int64_t g = l + (int64_t)j * P.elem_stride;
-                sv[l * lstride + pad + j] = vin[g];
-                sw[l * lstride + pad + j] = win[g];
-            }
-        } else {
-            for (int idx = tid; idx < cnt; idx += nthr) {
-                const int j = idx % L, l = idx / L;
-                const int64_t g = (int64_t)l * P.line_stride + (int64_t)j * P.elem_stride;
-                sv[l * lstride + pad + j] = vin[g];
-                sw[l * lstride + pad + j] = win[g];
-            }
-        }
-    } else {
+    if (INJECT) {
         // Deterministic injection.  The line's contributions arrive sorted by cell (point
         // order inside a cell).  The first wave of the line takes them in chunks of up to
         // 64; a segmented suffix sum over lanes of equal cell (fixed tree order) gives each
         // run's total to its first lane, which adds it to the cell.  Chunks of one line are
         // handled by one wave in program order, so the result does not depend on timing.
-        const int iw = lpl < 64 ? lpl : 64;
-        const bool injector = line_ok && sid < iw;
-        const int64_t W = F.size[0], H = F.size[1];
-        const int64_t line_in_field = wk.base / W + my_line;
-        uint32_t ls = 0, lc = 0;
-        if (injector) {
-            const int64_t gl = F.xline_off + line_in_field;
-            ls = line_start[gl];
-            lc = line_count[gl];
-        }
         int nch = injector ? (int)((lc + iw - 1) / iw) : 0;
         for (int d = 32; d >= 1; d >>= 1) nch = max(nch, __shfl_xor(nch, d));
         const int64_t ly = line_in_field % H, lz = line_in_field / H;
@@ -443,74 +571,45 @@ __global__ __launch_bounds__(kBlock) void k_fast_pass(
                 *pw = (float)((double)*pw + cw);
             }
         }
+        __syncthreads();
     }
-    __syncthreads();
 
-    // n-fold tailed box filter, in place
+    // n-fold tailed box filter, in place, one field at a time
     int seg = (L + lpl - 1) / lpl;
-    if (seg < kSegMax) seg |= 1;   // odd segment stride: fewer LDS bank conflicts
+    seg |= 1;   // odd segment stride: fewer LDS bank conflicts (host keeps seg <= kSegArr)
     const int s0 = sid * seg;
     int myseg = L - s0;
     myseg = myseg < 0 ? 0 : (myseg > seg ? seg : myseg);
+    const bool act = line_ok && myseg > 0;
     float *lv = sv + (size_t)(line_ok ? my_line : 0) * lstride + pad;
     float *lw = sw + (size_t)(line_ok ? my_line : 0) * lstride + pad;
     const double alpha = P.alpha;
     const int niter = F.num_iter;
-    float ra[kSegMax], rb[kSegMax];
+    float ra[kSegArr];
     for (int it = 0; it < niter; ++it) {
         const bool last = it == niter - 1;
-        const bool fin = last && final_pass;
         const double scale = last ? P.pscale : 1.0;
-        if (line_ok && myseg > 0) {
-            if (fin) slide<true>(lv, lw, s0, myseg, T, alpha, scale, F.thr_fast, off, ra, rb);
-            else slide<false>(lv, lw, s0, myseg, T, alpha, scale, 0.0, 0.0, ra, rb);
+        if (last && final_pass) {
+            if (act) slide_final(lv, lw, s0, myseg, T, alpha, scale, F.thr_fast, off, ra);
+            __syncthreads();
+            if (act) stage_store(lv, s0, myseg, ra);
+            __syncthreads();
+        } else {
+            if (act) slide1(lv, s0, myseg, T, alpha, scale, ra);
+            __syncthreads();
+            if (act) stage_store(lv, s0, myseg, ra);
+            if (act) slide1(lw, s0, myseg, T, alpha, scale, ra);
+            __syncthreads();
+            if (act) stage_store(lw, s0, myseg, ra);
+            __syncthreads();
         }
-        __syncthreads();
-        if (line_ok) {
-#pragma unroll
-            for (int j = 0; j < kSegMax; ++j) {
-                if (j < myseg) {
-                    lv[s0 + j] = ra[j];
-                    if (!fin) lw[s0 + j] = rb[j];
-                }
-            }
-        }
-        __syncthreads();
     }
 
-    // store (coalesced along the unit-stride dimension)
-    const int cnt = nl * L;
     if (final_pass) {
-        float *o = out + F.out_off + wk.base;
-        if (P.line_stride == 1) {
-            for (int idx = tid; idx < cnt; idx += nthr) {
-                const int l = idx % nl, j = idx / nl;
-                o[l + (int64_t)j * P.elem_stride] = sv[l * lstride + pad + j];
-            }
-        } else {
-            for (int idx = tid; idx < cnt; idx += nthr) {
-                const int j = idx % L, l = idx / L;
-                o[(int64_t)l * P.line_stride + (int64_t)j * P.elem_stride] = sv[l * lstride + pad + j];
-            }
-        }
+        tile_store(out + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
     } else {
-        float *vo = gv + F.cell_off + wk.base;
-        float *wo = gw + F.cell_off + wk.base;
-        if (P.line_stride == 1) {
-            for (int idx = tid; idx < cnt; idx += nthr) {
-                const int l = idx % nl, j = idx / nl;
-                const int64_t g = l + (int64_t)j * P.elem_stride;
-                vo[g] = sv[l * lstride + pad + j];
-                wo[g] = sw[l * lstride + pad + j];
-            }
-        } else {
-            for (int idx = tid; idx < cnt; idx += nthr) {
-                const int j = idx % L, l = idx / L;
-                const int64_t g = (int64_t)l * P.line_stride + (int64_t)j * P.elem_stride;
-                vo[g] = sv[l * lstride + pad + j];
-                wo[g] = sw[l * lstride + pad + j];
-            }
-        }
+        tile_store(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
+        tile_store(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
     }
 }
 

@@ -1,0 +1,149 @@
+"""
+Many fields in one batch: fields with uneven station counts, grid sizes, sigmas and
+iteration counts (time steps, variables) interpolated by one plan.
+
+The reference interpolates one field per ``barnes()`` call; a batch runs the same
+algorithm over all fields with every stage launched once for the whole batch, and
+optionally replays the launches as one captured hipGraph.
+
+    batch = BarnesBatch([dict(sigma=1.0, x0=(-26, 34), step=1/32, size=(2400, 1200), n=3490), ...],
+                        precision='fp32c')
+    outs = batch.interpolate(pts_list, val_list)            # numpy in, numpy out
+    batch.execute(d_pts, d_val, d_out, stream, graph=True)  # device pointers (fastbarnes.hip / torch)
+"""
+import ctypes
+from math import exp
+
+import numpy as np
+
+from fastbarnes import _lib
+from fastbarnes._lib import check, lib
+
+_METHOD = {'optimized_convolution': _lib.FB_METHOD_OPTIMIZED_CONVOLUTION, 'convolution': _lib.FB_METHOD_CONVOLUTION}
+
+
+def _axis3(x, dim, fill):
+    if isinstance(x, (list, tuple, np.ndarray)):
+        v = [float(a) for a in x]
+        if len(v) != dim:
+            raise RuntimeError('expected %d values per field, got %d' % (dim, len(v)))
+    else:
+        v = [float(x)] * dim
+    return v + [fill] * (3 - dim)
+
+
+class BarnesBatch:
+    """A plan for a fixed list of field descriptions.
+
+    Each field is a dict with keys sigma, x0, step, size (x, y[, z] order), n (number of
+    sample points) and optionally num_iter (4), max_dist (3.5) and method
+    ('optimized_convolution' or 'convolution').  All fields share one dimension.
+    Points of field f are rows [pt_offsets[f], pt_offsets[f] + n) of the batch's pts
+    array; its result occupies out_sizes[f] float32 elements at out_offsets[f].
+    """
+
+    def __init__(self, fields, precision='fp32c'):
+        if not fields:
+            raise RuntimeError('empty batch')
+        self.precision = precision
+        self.prec = _lib.PRECISIONS[precision]
+        arr = (_lib.FbField * len(fields))()
+        self.shapes = []
+        self.pt_offsets = []
+        self.out_offsets = []
+        pt = 0
+        out = 0
+        dim = None
+        for i, f in enumerate(fields):
+            size = f['size'] if isinstance(f['size'], (list, tuple, np.ndarray)) else (f['size'],)
+            d = len(size)
+            if dim is None:
+                dim = d
+            elif d != dim:
+                raise RuntimeError('all fields of a batch must have the same dimension')
+            a = arr[i]
+            a.dim = d
+            a.method = _METHOD[f.get('method', 'optimized_convolution')]
+            a.num_iter = int(f.get('num_iter', 4))
+            a.n = int(f['n'])
+            a.pt_offset = pt
+            a.out_offset = out
+            for m, s in enumerate(list(size) + [1] * (3 - d)):
+                a.size[m] = int(s)
+            for m, s in enumerate(_axis3(f['sigma'], d, 1.0)):
+                a.sigma[m] = s
+            for m, s in enumerate(_axis3(f['x0'], d, 0.0)):
+                a.x0[m] = s
+            for m, s in enumerate(_axis3(f['step'], d, 1.0)):
+                a.step[m] = s
+            a.max_dist_weight = exp(-float(f.get('max_dist', 3.5)) ** 2 / 2)
+            self.shapes.append(tuple(int(s) for s in size[::-1]))
+            self.pt_offsets.append(pt)
+            self.out_offsets.append(out)
+            pt += a.n
+            out += int(np.prod(size))
+        self.dim = dim
+        self.total_points = pt
+        self.total_out = out
+        self._fields = arr
+        self._plan = ctypes.c_void_p()
+        check(lib().fb_plan_create(ctypes.byref(self._plan), arr, len(fields), self.prec, 0))
+
+    @property
+    def nfields(self):
+        return len(self.shapes)
+
+    def info(self):
+        cells = ctypes.c_int64()
+        ws = ctypes.c_int64()
+        check(lib().fb_plan_info(self._plan, ctypes.byref(cells), ctypes.byref(ws)))
+        return dict(grid_points=cells.value, workspace_bytes=ws.value)
+
+    def execute(self, d_pts, d_val, d_out, stream, graph=False):
+        """Runs the batch on device buffers (integer device addresses) on `stream` (a hipStream_t
+        address); asynchronous.  graph=True captures the launches once per buffer/stream tuple
+        and replays the captured hipGraph afterwards."""
+        fn = lib().fb_plan_execute_graph if graph else lib().fb_plan_execute
+        check(fn(self._plan, d_pts, d_val, d_out, stream))
+
+    def set_timing(self, enable):
+        check(lib().fb_plan_set_timing(self._plan, int(bool(enable))))
+
+    def stage_times(self):
+        """{stage: ms} of the last execute() run with timing enabled (HIP events on its stream)."""
+        ms = (ctypes.c_float * 16)()
+        n = ctypes.c_int()
+        names = ctypes.create_string_buffer(512)
+        check(lib().fb_plan_stage_times(self._plan, ms, 16, ctypes.byref(n), names, 512))
+        keys = names.value.decode().split(',') if n.value else []
+        return dict(zip(keys, [ms[i] for i in range(n.value)]))
+
+    def interpolate(self, pts_list, val_list, stream=None):
+        """Host convenience: uploads the fields' points and values, runs the batch and
+        returns one float32 array per field (shape size[::-1])."""
+        from fastbarnes import hip
+        if len(pts_list) != self.nfields or len(val_list) != self.nfields:
+            raise RuntimeError('expected %d fields' % self.nfields)
+        pts = np.concatenate([np.asarray(p, dtype=np.float64).reshape(len(p), -1) for p in pts_list])
+        val = np.concatenate([np.asarray(v, dtype=np.float64) for v in val_list])
+        if pts.shape[0] != self.total_points or val.shape[0] != self.total_points:
+            raise RuntimeError('point counts differ from the plan')
+        d_pts = hip.DeviceArray.from_numpy(pts)
+        d_val = hip.DeviceArray.from_numpy(val)
+        d_out = hip.DeviceArray((self.total_out,), np.float32)
+        s = stream or hip.Stream()
+        self.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr)
+        s.synchronize()
+        flat = d_out.numpy()
+        return [flat[o:o + int(np.prod(sh))].reshape(sh) for o, sh in zip(self.out_offsets, self.shapes)]
+
+    def close(self):
+        if self._plan:
+            lib().fb_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

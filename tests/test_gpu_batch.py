@@ -1,0 +1,109 @@
+"""
+GPU tests of batched plans (marked `gpu`): uneven fields in one plan, eager and hipGraph
+replays.  Each field of a batch must equal the single-field drop-in result bit for bit
+(both precisions use the same kernels and per-field work decomposition), and the
+single-field FP64 results equal the oracle (tests/test_gpu_parity.py).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _fields(seed=0):
+    from fastbarnes import workloads
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(5):
+        W = int(rng.choice([150, 300, 600]))
+        H = W // 2
+        n = int(rng.integers(200, 4000))
+        pts = workloads.stations_2d(n, seed * 100 + i)
+        out.append(dict(pts=pts, val=workloads.values_sinusoids(pts, i), sigma=float(rng.uniform(0.5, 2.0)),
+                        x0=np.array([-26.0, 34.0]), step=75.0 / W, size=(W, H), num_iter=int(rng.integers(3, 7)),
+                        max_dist=3.5))
+    return out
+
+
+def _desc(c):
+    return dict(sigma=c['sigma'], x0=c['x0'], step=c['step'], size=c['size'], n=len(c['val']),
+                num_iter=c['num_iter'], max_dist=c['max_dist'])
+
+
+def _bitwise(a, b):
+    np.testing.assert_array_equal(np.isnan(a), np.isnan(b))
+    ok = ~np.isnan(a)
+    np.testing.assert_array_equal(a[ok].view(np.uint32), b[ok].view(np.uint32))
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32c'])
+def test_batch_equals_single_calls(precision):
+    from fastbarnes.batched import BarnesBatch
+    from fastbarnes.interpolation import barnes
+    fs = _fields()
+    batch = BarnesBatch([_desc(c) for c in fs], precision=precision)
+    outs = batch.interpolate([c['pts'] for c in fs], [c['val'] for c in fs])
+    for c, o in zip(fs, outs):
+        ref = barnes(c['pts'], c['val'], c['sigma'], c['x0'], c['step'], c['size'], num_iter=c['num_iter'],
+                     max_dist=c['max_dist'], precision=precision)
+        _bitwise(o, ref)
+    info = batch.info()
+    assert info['grid_points'] == sum(c['size'][0] * c['size'][1] for c in fs)
+
+
+def test_graph_replays_match_eager():
+    from fastbarnes import hip
+    from fastbarnes.batched import BarnesBatch
+    fs = _fields(1)
+    batch = BarnesBatch([_desc(c) for c in fs], precision='fp32c')
+    pts = np.concatenate([c['pts'] for c in fs])
+    val = np.concatenate([c['val'] for c in fs])
+    d_pts, d_val = hip.DeviceArray.from_numpy(pts), hip.DeviceArray.from_numpy(val)
+    d_out = hip.DeviceArray((batch.total_out,), np.float32)
+    s = hip.Stream()
+    batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr, graph=False)
+    s.synchronize()
+    eager = d_out.numpy()
+    for _ in range(4):
+        d_out.upload(np.full(batch.total_out, 12345.0, dtype=np.float32))   # stale data must be overwritten
+        batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr, graph=True)
+        s.synchronize()
+        _bitwise(d_out.numpy(), eager)
+    # new values through the same captured graph (buffers reused in place)
+    d_val.upload(val * 2.0 - 1.0)
+    batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr, graph=True)
+    s.synchronize()
+    g2 = d_out.numpy()
+    batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr, graph=False)
+    s.synchronize()
+    _bitwise(g2, d_out.numpy())
+
+
+def test_stage_timing():
+    from fastbarnes import hip
+    from fastbarnes.batched import BarnesBatch
+    fs = _fields(2)[:2]
+    batch = BarnesBatch([_desc(c) for c in fs], precision='fp32c')
+    pts = np.concatenate([c['pts'] for c in fs])
+    val = np.concatenate([c['val'] for c in fs])
+    d_pts, d_val = hip.DeviceArray.from_numpy(pts), hip.DeviceArray.from_numpy(val)
+    d_out = hip.DeviceArray((batch.total_out,), np.float32)
+    s = hip.Stream()
+    batch.set_timing(True)
+    batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr)
+    t = batch.stage_times()
+    assert list(t) == ['sort', 'pass_x', 'pass_y'] and all(v > 0 for v in t.values())
+
+
+def test_batch_3d_fp64_equals_single():
+    from fastbarnes import workloads
+    from fastbarnes.batched import BarnesBatch
+    from fastbarnes.interpolation import barnes
+    fs = [workloads.config4(seed=s, n=5000 * (s + 1), size=(64, 48, 24)) for s in range(3)]
+    for prec in ('fp64', 'fp32c'):
+        batch = BarnesBatch([_desc(c) for c in fs], precision=prec)
+        outs = batch.interpolate([c['pts'] for c in fs], [c['val'] for c in fs])
+        for c, o in zip(fs, outs):
+            ref = barnes(c['pts'], c['val'], c['sigma'], c['x0'], c['step'], c['size'], num_iter=c['num_iter'],
+                         max_dist=c['max_dist'], precision=prec)
+            _bitwise(o, ref)
