@@ -22,9 +22,10 @@ const char *last_error() { return g_err.c_str(); }
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kSegMax = 40;              // maximum segment length per lane in the fast passes
-                                         // (odd-rounded to 41 = register staging of the kernel)
+constexpr int kBlock = 256;             // lanes per field of a fast-pass workgroup (the
+                                         // workgroup holds kBlock v workers + kBlock w workers)
+constexpr int kFastThreads = 2 * kBlock;
+constexpr int kSegLen = 41;              // elements per lane in the fast passes (kSegArr of the kernel)
 constexpr size_t kLdsBudget = 80 * 1024; // two workgroups per CU
 constexpr size_t kLdsMax = 160 * 1024;
 
@@ -192,19 +193,21 @@ hipStream_t lib_stream(int device)
 // lanes per line / lines per workgroup of a fast line pass
 int pass_geometry(int64_t L, int T, int &lpl, int &lines, size_t &lds)
 {
-    const int64_t need = (L + kSegMax - 1) / kSegMax;
+    const int64_t need = (L + kSegLen - 1) / kSegLen;
     lpl = 1;
     while (lpl < need) lpl <<= 1;
     if (lpl > kBlock)
         return fail(FB_ERR_UNSUPPORTED, "line length " + std::to_string(L) + " exceeds the FP32C limit of " +
-                                            std::to_string(kBlock * kSegMax));
+                                            std::to_string(kBlock * kSegLen));
     lines = kBlock / lpl;
     const size_t per_line = (size_t)(L + 2 * (int64_t)T + 2) * 2 * sizeof(float);
-    while (lines > 1 && lines * per_line > kLdsBudget) {
+    // branch-free slides read up to kSegLen-1 floats past the last w line: reserve 64 floats
+    const size_t reserve = 64 * sizeof(float);
+    while (lines > 1 && lines * per_line + reserve > kLdsBudget) {
         lines >>= 1;
         lpl <<= 1;
     }
-    lds = round_bytes(lines * per_line);
+    lds = round_bytes(lines * per_line + reserve);
     if (lds > kLdsMax)
         return fail(FB_ERR_UNSUPPORTED, "line of length " + std::to_string(L) + " with half kernel " +
                                             std::to_string(T) + " does not fit the local data share");
@@ -543,7 +546,7 @@ int plan_run(fb_plan *P, const double *pts, const double *val, float *out, hipSt
         for (int a = 0; a < P->dim; ++a) {
             const int fin = a == P->dim - 1;
             if ((rc = launch_fast_pass(a == 0, P->d_fields, P->d_offsets, P->d_work[a], P->nwork[a], P->d_pass[a],
-                                       kBlock, P->lds[a], ks, vs, P->d_line_start, P->d_line_count, pts, val,
+                                       kFastThreads, P->lds[a], ks, vs, P->d_line_start, P->d_line_count, pts, val,
                                        P->d_v, P->d_w, out, fin, s)))
                 return rc;
             stage_mark(P, s, pn[a]);
@@ -901,7 +904,7 @@ int fb_accumulate_lines(int tail, int precision, int64_t nlines, int64_t len, in
         if ((rc = launch_zero(d_off, 8, s))) return rc;
         if ((rc = launch_f64_to_f32(d_io, d_v, cells, s))) return rc;
         if ((rc = launch_zero(d_w, cells * 4, s))) return rc;
-        if ((rc = launch_fast_pass(false, d_f, d_off, d_work, (int)work.size(), d_pf, kBlock, lds, nullptr, nullptr,
+        if ((rc = launch_fast_pass(false, d_f, d_off, d_work, (int)work.size(), d_pf, kFastThreads, lds, nullptr, nullptr,
                                    nullptr, nullptr, nullptr, nullptr, d_v, d_w, nullptr, 0, s)))
             return rc;
         if ((rc = launch_f32_to_f64(d_v, d_io, cells, 1.0, s))) return rc;
@@ -992,7 +995,7 @@ int fb_convolve_grid(int tail, int precision, int dim, const int64_t *size, cons
             if ((rc = sc.get(&d_work, work.size())) || (rc = sc.get(&d_pf, 1))) return rc;
             FB_HIP_CHECK(hipMemcpy(d_work, work.data(), work.size() * sizeof(PassWork), hipMemcpyHostToDevice));
             FB_HIP_CHECK(hipMemcpy(d_pf, pf.data(), sizeof(PassField), hipMemcpyHostToDevice));
-            if ((rc = launch_fast_pass(false, d_f, d_off, d_work, (int)work.size(), d_pf, kBlock, lds, nullptr,
+            if ((rc = launch_fast_pass(false, d_f, d_off, d_work, (int)work.size(), d_pf, kFastThreads, lds, nullptr,
                                        nullptr, nullptr, nullptr, nullptr, nullptr, d_v, d_w, nullptr, 0, s)))
                 return rc;
             FB_HIP_CHECK(hipStreamSynchronize(s));   // host vectors above go out of scope

@@ -20,7 +20,6 @@ namespace fb {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kSegMax = 64;   // maximum elements of one line segment held by one lane
 
 __device__ __forceinline__ int find_seg(const int64_t *__restrict__ prefix, int n, int64_t i)
 {
@@ -305,73 +304,102 @@ __global__ __launch_bounds__(kBlock) void k_mask(const FieldDev *__restrict__ fi
 // by (1/(K+2*alpha))^n and, on the last pass, forms the masked FP64 quotient
 // directly from the FP64 window sums of both fields.
 
-constexpr int kSegArr = 41;   // register staging per lane: segments of up to 41 elements
+constexpr int kSegArr = 41;   // elements per lane segment (odd: conflict-free lane stride)
+constexpr int kFastBlock = 512;   // threads of a fast-pass workgroup: 256 per field
+#ifndef FB_ADD_BLOCK
+#define FB_ADD_BLOCK 4
+#endif
+constexpr int kBlk = FB_ADD_BLOCK;                    // add-stream block (LDS loads in flight)
+constexpr int kNBlk = (kSegArr + kBlk - 1) / kBlk;    // blocks covering the kSegArr steps
 
-// one iteration of the tailed box filter over the lane's segment [s, s+n) of line `lin`
-__device__ __forceinline__ void slide1(const float *__restrict__ lin, int s, int n, int T, double alpha, double scale,
-                                       float (&ra)[kSegArr])
+// One iteration of the tailed box filter over the lane's segment starting at s of one
+// field's line `lin`.  The subtract stream lin[s-T+j] (j < kSegArr) is read in one burst
+// into registers and doubles as the first part of the initial window sum lin[s-T .. s+T];
+// the add stream lin[s+T+1+j] is read in blocks of kBlk with one block of lookahead, so
+// many LDS loads are in flight.  All kSegArr steps run unconditionally; steps past the
+// lane's segment read in-bounds LDS (next line or the tail reserve) and their outputs are
+// discarded by stage_store.  Results replace sb[j] (dead after step j: its value lives on
+// in p), so the register array doubles as the staging buffer of the in-place update.
+//   SCALE: multiply the results by `scale` (last iteration of a pass).
+template <bool SCALE>
+__device__ __forceinline__ void slide1(const float *__restrict__ lin, int s, int T, double alpha, double scale,
+                                       float (&sb)[kSegArr])
 {
-    double a = 0.0;
-    for (int i = s - T; i <= s + T; ++i) a += (double)lin[i];
-    double p = (double)lin[s - T - 1];
-    const float *addp = lin + s + T + 1;
+    const int K = 2 * T + 1;
     const float *subp = lin + s - T;
 #pragma unroll
-    for (int j = 0; j < kSegArr; ++j) {
-        if (j < n) {
-            const double ad = (double)addp[j];
-            const double sb = (double)subp[j];
-            ra[j] = (float)(fma(alpha, p + ad, a) * scale);
-            a += ad - sb;
-            p = sb;
-        }
-    }
-}
-
-// last iteration of the last pass: both fields, masked quotient + offset
-__device__ __forceinline__ void slide_final(const float *__restrict__ lv, const float *__restrict__ lw, int s, int n,
-                                            int T, double alpha, double scale, double thr, double off,
-                                            float (&ra)[kSegArr])
-{
-    double av = 0.0, aw = 0.0;
-    for (int i = s - T; i <= s + T; ++i) {
-        av += (double)lv[i];
-        aw += (double)lw[i];
-    }
-    double pv = (double)lv[s - T - 1];
-    double pw = (double)lw[s - T - 1];
+    for (int j = 0; j < kSegArr; ++j) sb[j] = subp[j];
+    const float p0 = lin[s - T - 1];
+    // window sum: the first min(K, kSegArr) terms are sb[], the rest (K > kSegArr) from LDS
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = kSegArr; i < K; i += 4) {
+        float x[4];
 #pragma unroll
-    for (int j = 0; j < kSegArr; ++j) {
-        if (j < n) {
-            const int k = s + j;
-            const double addv = (double)lv[k + T + 1];
-            const double subv = (double)lv[k - T];
-            const double addw = (double)lw[k + T + 1];
-            const double subw = (double)lw[k - T];
-            const double ov = fma(alpha, pv + addv, av) * scale;
-            const double ow = fma(alpha, pw + addw, aw) * scale;
-            av += addv - subv;
-            aw += addw - subw;
-            pv = subv;
-            pw = subw;
-            ra[j] = (ow < thr) ? __builtin_nanf("") : (float)(ov / ow + off);
-        }
+        for (int u = 0; u < 4; ++u) x[u] = (i + u < K) ? subp[i + u] : 0.f;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] += (double)x[u];
     }
-}
-
-__device__ __forceinline__ void stage_store(float *__restrict__ lin, int s, int n, const float (&ra)[kSegArr])
-{
 #pragma unroll
     for (int j = 0; j < kSegArr; ++j)
-        if (j < n) lin[s + j] = ra[j];
+        if (j < K) v[j & 3] += (double)sb[j];
+    double a = (v[0] + v[1]) + (v[2] + v[3]);
+    double p = (double)p0;
+    const float *addp = lin + s + T + 1;
+    float cur[kBlk], nxt[kBlk];
+#pragma unroll
+    for (int u = 0; u < kBlk; ++u) cur[u] = addp[u];
+#pragma unroll
+    for (int b = 0; b < kNBlk; ++b) {
+        if (b + 1 < kNBlk) {
+#pragma unroll
+            for (int u = 0; u < kBlk; ++u)
+                if ((b + 1) * kBlk + u < kSegArr) nxt[u] = addp[(b + 1) * kBlk + u];
+        }
+#pragma unroll
+        for (int u = 0; u < kBlk; ++u) {
+            const int j = b * kBlk + u;
+            if (j < kSegArr) {
+                const double ad = (double)cur[u];
+                const double sv = (double)sb[j];
+                double o = fma(alpha, p + ad, a);
+                a += ad - sv;
+                p = sv;
+                if (SCALE) o *= scale;
+                // the empty asm pins each float32 result at its step; otherwise the compiler
+                // sinks all conversions to the store and keeps kSegArr doubles live
+                float r = (float)o;
+                asm volatile("" : "+v"(r));
+                sb[j] = r;
+            }
+        }
+        if (b + 1 < kNBlk) {
+#pragma unroll
+            for (int u = 0; u < kBlk; ++u) cur[u] = nxt[u];
+        }
+    }
+}
+
+__device__ __forceinline__ void stage_store(float *__restrict__ lin, int s, int n, const float (&sb)[kSegArr])
+{
+    if (n == kSegArr) {
+#pragma unroll
+        for (int j = 0; j < kSegArr; ++j) lin[s + j] = sb[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < kSegArr; ++j)
+            if (j < n) lin[s + j] = sb[j];
+    }
 }
 
 __device__ __forceinline__ bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 
-// global tile -> LDS.  Tile: nl lines of length L; element (l, j) at g[l*ls + j*es] goes to
-// lds[l*lstride + j].  Either ls == 1 (columns, rows of nl contiguous floats) or es == 1
-// (rows of L contiguous floats).  Float4 accesses when the layout allows.
-__device__ __forceinline__ void tile_load(const float *__restrict__ g, int64_t ls, int64_t es, int nl, int L,
+// global tile <-> LDS.  Tile: nl lines of length L; element (l, j) at g[l*ls + j*es] maps
+// to lds[l*lstride + j].  Either ls == 1 (columns: rows of nl contiguous floats) or es == 1
+// (rows of L contiguous floats); float4 global accesses when the layout allows.  All
+// threads of the workgroup take part.  Pointers advance by increments (no 64-bit index
+// products per element).
+template <bool LOAD>
+__device__ __forceinline__ void tile_copy(float *__restrict__ g, int64_t ls, int64_t es, int nl, int L,
                                           float *__restrict__ lds, int lstride)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -382,14 +410,20 @@ __device__ __forceinline__ void tile_load(const float *__restrict__ g, int64_t l
             const int q = tid % nq;
             const int r0 = tid / nq;
             if (r0 < rstep) {
+                float *gp = g + (int64_t)r0 * es + 4 * q;
+                const int64_t gstep = (int64_t)rstep * es;
+                float *d = lds + (4 * q) * lstride + r0;
 #pragma unroll 4
-                for (int r = r0; r < L; r += rstep) {
-                    const float4 x = *reinterpret_cast<const float4 *>(g + (int64_t)r * es + 4 * q);
-                    float *d = lds + (4 * q) * lstride + r;
-                    d[0] = x.x;
-                    d[lstride] = x.y;
-                    d[2 * lstride] = x.z;
-                    d[3 * lstride] = x.w;
+                for (int r = r0; r < L; r += rstep, gp += gstep, d += rstep) {
+                    if (LOAD) {
+                        const float4 x = *reinterpret_cast<const float4 *>(gp);
+                        d[0] = x.x;
+                        d[lstride] = x.y;
+                        d[2 * lstride] = x.z;
+                        d[3 * lstride] = x.w;
+                    } else {
+                        *reinterpret_cast<float4 *>(gp) = make_float4(d[0], d[lstride], d[2 * lstride], d[3 * lstride]);
+                    }
                 }
             }
         } else {
@@ -397,80 +431,57 @@ __device__ __forceinline__ void tile_load(const float *__restrict__ g, int64_t l
             const int l = tid % nl;
             const int r0 = tid / nl;
             if (r0 < rstep) {
+                float *gp = g + (int64_t)r0 * es + l;
+                const int64_t gstep = (int64_t)rstep * es;
+                float *d = lds + l * lstride + r0;
 #pragma unroll 4
-                for (int r = r0; r < L; r += rstep) lds[l * lstride + r] = g[(int64_t)r * es + l];
+                for (int r = r0; r < L; r += rstep, gp += gstep, d += rstep) {
+                    if (LOAD) *d = *gp;
+                    else *gp = *d;
+                }
             }
         }
     } else {
         if ((L & 3) == 0 && (ls & 3) == 0 && aligned16(g)) {
             const int Lq = L >> 2;
             for (int l = 0; l < nl; ++l) {
+                float4 *gp = reinterpret_cast<float4 *>(g + l * ls) + tid;
+                float4 *d = reinterpret_cast<float4 *>(lds + l * lstride) + tid;
 #pragma unroll 4
-                for (int q = tid; q < Lq; q += nthr) {
-                    const float4 x = *reinterpret_cast<const float4 *>(g + l * ls + 4 * q);
-                    float *d = lds + l * lstride + 4 * q;
-                    d[0] = x.x;
-                    d[1] = x.y;
-                    d[2] = x.z;
-                    d[3] = x.w;
+                for (int q = tid; q < Lq; q += nthr, gp += nthr, d += nthr) {
+                    // LDS rows start at pad offsets: float4 LDS access only when 16-byte aligned
+                    float *dd = reinterpret_cast<float *>(d);
+                    if (LOAD) {
+                        const float4 x = *gp;
+                        dd[0] = x.x;
+                        dd[1] = x.y;
+                        dd[2] = x.z;
+                        dd[3] = x.w;
+                    } else {
+                        *gp = make_float4(dd[0], dd[1], dd[2], dd[3]);
+                    }
                 }
             }
         } else {
-            for (int l = 0; l < nl; ++l)
-#pragma unroll 4
-                for (int j = tid; j < L; j += nthr) lds[l * lstride + j] = g[l * ls + (int64_t)j * es];
-        }
-    }
-}
-
-// LDS -> global tile (inverse of tile_load)
-__device__ __forceinline__ void tile_store(float *__restrict__ g, int64_t ls, int64_t es, int nl, int L,
-                                           const float *__restrict__ lds, int lstride)
-{
-    const int tid = threadIdx.x, nthr = blockDim.x;
-    if (ls == 1) {
-        if ((nl & 3) == 0 && (es & 3) == 0 && aligned16(g)) {
-            const int nq = nl >> 2;
-            const int rstep = nthr / nq;
-            const int q = tid % nq;
-            const int r0 = tid / nq;
-            if (r0 < rstep) {
-#pragma unroll 4
-                for (int r = r0; r < L; r += rstep) {
-                    const float *d = lds + (4 * q) * lstride + r;
-                    *reinterpret_cast<float4 *>(g + (int64_t)r * es + 4 * q) =
-                        make_float4(d[0], d[lstride], d[2 * lstride], d[3 * lstride]);
-                }
-            }
-        } else {
-            const int rstep = nthr / nl;
-            const int l = tid % nl;
-            const int r0 = tid / nl;
-            if (r0 < rstep) {
-#pragma unroll 4
-                for (int r = r0; r < L; r += rstep) g[(int64_t)r * es + l] = lds[l * lstride + r];
-            }
-        }
-    } else {
-        if ((L & 3) == 0 && (ls & 3) == 0 && aligned16(g)) {
-            const int Lq = L >> 2;
             for (int l = 0; l < nl; ++l) {
+                float *gp = g + l * ls + (int64_t)tid * es;
+                const int64_t gstep = (int64_t)nthr * es;
+                float *d = lds + l * lstride + tid;
 #pragma unroll 4
-                for (int q = tid; q < Lq; q += nthr) {
-                    const float *d = lds + l * lstride + 4 * q;
-                    *reinterpret_cast<float4 *>(g + l * ls + 4 * q) = make_float4(d[0], d[1], d[2], d[3]);
+                for (int j = tid; j < L; j += nthr, gp += gstep, d += nthr) {
+                    if (LOAD) *d = *gp;
+                    else *gp = *d;
                 }
             }
-        } else {
-            for (int l = 0; l < nl; ++l)
-#pragma unroll 4
-                for (int j = tid; j < L; j += nthr) g[l * ls + (int64_t)j * es] = lds[l * lstride + j];
         }
     }
 }
 
+// One workgroup = kFastBlock threads: the first half works on the v field, the second half
+// on the w field of the same nlines lines (each half: lines_cap = 256/lpl lines of lpl
+// lanes).  LDS holds the v lines, then the w lines, each padded by T+1 zeros per side.
 template <bool INJECT>
-__global__ __launch_bounds__(kBlock, 2) void k_fast_pass(
+__global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     const FieldDev *__restrict__ fields, const double *__restrict__ offsets, const PassWork *__restrict__ work,
     const PassField *__restrict__ passes, const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
     const uint32_t *__restrict__ line_start, const uint32_t *__restrict__ line_count, const double *__restrict__ pts,
@@ -483,17 +494,21 @@ __global__ __launch_bounds__(kBlock, 2) void k_fast_pass(
     const PassField P = passes[wk.field];
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
+    const int half = nthr >> 1;
+    const int fsel = tid >= half;            // 0: v worker, 1: w worker
+    const int ltid = tid - fsel * half;
     const int L = (int)P.len;
     const int T = P.T;
     const int lpl = P.lpl;
-    const int lines_cap = nthr / lpl;
+    const int lines_cap = half / lpl;
     const int pad = T + 1;
     const int lstride = L + 2 * pad;
     const int nl = wk.nlines;
     float *sv = reinterpret_cast<float *>(smem);
     float *sw = sv + (size_t)lines_cap * lstride;
-    const int my_line = tid / lpl;
-    const int sid = tid - my_line * lpl;
+    float *mine = fsel ? sw : sv;
+    const int my_line = ltid / lpl;
+    const int sid = ltid - my_line * lpl;
     const bool line_ok = my_line < nl;
     const double off = offsets[wk.field];
 
@@ -510,30 +525,30 @@ __global__ __launch_bounds__(kBlock, 2) void k_fast_pass(
     }
 
     if (INJECT) {
-        // zero the lines and their pads
         float4 *z = reinterpret_cast<float4 *>(smem);
-        const int nz = (2 * lines_cap * lstride) >> 2;
+        const int tot = 2 * lines_cap * lstride;
+        const int nz = tot >> 2;
         for (int i = tid; i < nz; i += nthr) z[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int i = (nz << 2) + tid; i < 2 * lines_cap * lstride; i += nthr) sv[i] = 0.f;
+        for (int i = (nz << 2) + tid; i < tot; i += nthr) sv[i] = 0.f;
     } else {
-        // zero the pads only, then load the lines
         for (int i = tid; i < lines_cap * 2 * pad; i += nthr) {
             const int l = i / (2 * pad), r = i - l * 2 * pad;
             const int o = l * lstride + (r < pad ? r : L + r);
             sv[o] = 0.f;
             sw[o] = 0.f;
         }
-        tile_load(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
-        tile_load(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
+        tile_copy<true>(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
+        tile_copy<true>(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
     }
     __syncthreads();
 
     if (INJECT) {
         // Deterministic injection.  The line's contributions arrive sorted by cell (point
-        // order inside a cell).  The first wave of the line takes them in chunks of up to
-        // 64; a segmented suffix sum over lanes of equal cell (fixed tree order) gives each
-        // run's total to its first lane, which adds it to the cell.  Chunks of one line are
-        // handled by one wave in program order, so the result does not depend on timing.
+        // order inside a cell).  The first wave of the line's workers takes them in chunks of
+        // up to 64; a segmented suffix sum over lanes of equal cell (fixed tree order) gives
+        // each run's total to its first lane, which adds it to the cell of its own field.
+        // Chunks of one line are handled by one wave in program order, so the result does
+        // not depend on timing.
         int nch = injector ? (int)((lc + iw - 1) / iw) : 0;
         for (int d = 32; d >= 1; d >>= 1) nch = max(nch, __shfl_xor(nch, d));
         const int64_t ly = line_in_field % H, lz = line_in_field / H;
@@ -541,7 +556,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_fast_pass(
             const uint32_t e = (uint32_t)ch * iw + sid;
             const bool valid = injector && e < lc;
             int x = -1;
-            double cv = 0.0, cw = 0.0;
+            double c = 0.0;
             if (valid) {
                 const uint32_t key = keys[ls + e];
                 const uint32_t p = vals[ls + e];
@@ -551,65 +566,64 @@ __global__ __launch_bounds__(kBlock, 2) void k_fast_pass(
                 PointPos pp;
                 point_pos(F, pts + (int64_t)p * F.dim, pp);
                 const double w = corner_weight(F.dim, pp, x - pp.i[0], ly - pp.i[1], lz - pp.i[2]);
-                cv = w * (val[p] - off);
-                cw = w;
+                c = fsel ? w : w * (val[p] - off);
             }
             for (int d = 1; d < iw; d <<= 1) {
-                const double ov = __shfl_down(cv, d, iw);
-                const double ow = __shfl_down(cw, d, iw);
+                const double oc = __shfl_down(c, d, iw);
                 const int ox = __shfl_down(x, d, iw);
-                if (sid + d < iw && ox == x) {
-                    cv += ov;
-                    cw += ow;
-                }
+                if (sid + d < iw && ox == x) c += oc;
             }
             const int px = __shfl_up(x, 1, iw);
             if (valid && x >= 0 && (sid == 0 || px != x)) {
-                float *pv = sv + my_line * lstride + pad + x;
-                float *pw = sw + my_line * lstride + pad + x;
-                *pv = (float)((double)*pv + cv);
-                *pw = (float)((double)*pw + cw);
+                float *cell = mine + my_line * lstride + pad + x;
+                *cell = (float)((double)*cell + c);
             }
         }
         __syncthreads();
     }
 
-    // n-fold tailed box filter, in place, one field at a time
-    int seg = (L + lpl - 1) / lpl;
-    seg |= 1;   // odd segment stride: fewer LDS bank conflicts (host keeps seg <= kSegArr)
-    const int s0 = sid * seg;
+    // n-fold tailed box filter, in place.  Every lane owns kSegArr consecutive elements;
+    // the host picks lpl with lpl*kSegArr >= L and reserves 64 floats past the w lines so
+    // that the branch-free slides never read out of bounds.
+    const int s0 = sid * kSegArr;
     int myseg = L - s0;
-    myseg = myseg < 0 ? 0 : (myseg > seg ? seg : myseg);
+    myseg = myseg < 0 ? 0 : (myseg > kSegArr ? kSegArr : myseg);
     const bool act = line_ok && myseg > 0;
-    float *lv = sv + (size_t)(line_ok ? my_line : 0) * lstride + pad;
-    float *lw = sw + (size_t)(line_ok ? my_line : 0) * lstride + pad;
+    float *ln = mine + (size_t)(line_ok ? my_line : 0) * lstride + pad;
     const double alpha = P.alpha;
     const int niter = F.num_iter;
-    float ra[kSegArr];
-    for (int it = 0; it < niter; ++it) {
-        const bool last = it == niter - 1;
-        const double scale = last ? P.pscale : 1.0;
-        if (last && final_pass) {
-            if (act) slide_final(lv, lw, s0, myseg, T, alpha, scale, F.thr_fast, off, ra);
-            __syncthreads();
-            if (act) stage_store(lv, s0, myseg, ra);
-            __syncthreads();
-        } else {
-            if (act) slide1(lv, s0, myseg, T, alpha, scale, ra);
-            __syncthreads();
-            if (act) stage_store(lv, s0, myseg, ra);
-            if (act) slide1(lw, s0, myseg, T, alpha, scale, ra);
-            __syncthreads();
-            if (act) stage_store(lw, s0, myseg, ra);
-            __syncthreads();
-        }
+    float sb[kSegArr];
+    for (int it = 0; it + 1 < niter; ++it) {
+        if (act) slide1<false>(ln, s0, T, alpha, 1.0, sb);
+        __syncthreads();
+        if (act) stage_store(ln, s0, myseg, sb);
+        __syncthreads();
     }
+    if (act) slide1<true>(ln, s0, T, alpha, P.pscale, sb);
+    __syncthreads();
+    if (act) stage_store(ln, s0, myseg, sb);
+    __syncthreads();
 
     if (final_pass) {
-        tile_store(out + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
+        // masked quotient + offset (interpolation.py:367, :428-430), in place in the v lines
+        // quotient: float32 reciprocal refined by one FP64 Newton step (relative error ~1e-14;
+        // w is normalised, so it lies well inside the float32 range)
+        const double thr = F.thr_fast;
+        for (int l = 0; l < nl; ++l)
+            for (int j = tid; j < L; j += nthr) {
+                const int o = l * lstride + pad + j;
+                const float wf = sw[o];
+                const double w = (double)wf, v = (double)sv[o];
+                const double r0 = (double)__builtin_amdgcn_rcpf(wf);
+                double q = v * r0;
+                q = fma(r0, fma(-q, w, v), q);
+                sv[o] = (w < thr) ? __builtin_nanf("") : (float)(q + off);
+            }
+        __syncthreads();
+        tile_copy<false>(out + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
     } else {
-        tile_store(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
-        tile_store(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
+        tile_copy<false>(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
+        tile_copy<false>(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
     }
 }
 
@@ -808,7 +822,7 @@ int launch_fast_pass(bool inject, const FieldDev *d_fields, const double *d_offs
                      hipStream_t s)
 {
     if (nwork <= 0) return FB_OK;
-    if (threads != kBlock) {
+    if (threads != kFastBlock) {
         set_error("fast pass: unsupported workgroup size");
         return FB_ERR_UNSUPPORTED;
     }

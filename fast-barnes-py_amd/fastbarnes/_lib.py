@@ -9,7 +9,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, 'libfastbarnes.so')
+# FASTBARNES_LIB selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get('FASTBARNES_LIB') or os.path.join(_HERE, 'libfastbarnes.so')
 
 FB_OK = 0
 FB_ERR_INVALID_ARGUMENT = 1
