@@ -332,16 +332,20 @@ __device__ __forceinline__ void slide1(const float *__restrict__ lin, int s, int
     const float p0 = lin[s - T - 1];
     // window sum: the first min(K, kSegArr) terms are sb[], the rest (K > kSegArr) from LDS
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = kSegArr; i < K; i += 4) {
-        float x[4];
+    if (K >= kSegArr) {
+        for (int i = kSegArr; i < K; i += 4) {
+            float x[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) x[u] = (i + u < K) ? subp[i + u] : 0.f;
+            for (int u = 0; u < 4; ++u) x[u] = (i + u < K) ? subp[i + u] : 0.f;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] += (double)x[u];
+            for (int u = 0; u < 4; ++u) v[u] += (double)x[u];
+        }
+#pragma unroll
+        for (int j = 0; j < kSegArr; ++j) v[j & 3] += (double)sb[j];
+    } else {
+        // short kernels (2T+1 < kSegArr): direct sum from LDS
+        for (int i = 0; i < K; ++i) v[i & 3] += (double)subp[i];
     }
-#pragma unroll
-    for (int j = 0; j < kSegArr; ++j)
-        if (j < K) v[j & 3] += (double)sb[j];
     double a = (v[0] + v[1]) + (v[2] + v[3]);
     double p = (double)p0;
     const float *addp = lin + s + T + 1;
