@@ -79,6 +79,9 @@ def main():
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-extras', action='store_true', help='skip accuracy and FP64-path legs')
+    ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
+                    help='after the run, one device-to-device copy of MIB MiB (a plain copy kernel of known '
+                         'size that scales HBM byte counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: this build has no multi-GPU path (fields shard across GPUs '
@@ -214,6 +217,13 @@ def main():
         'wall_s_timed': round(wall, 4),
     }
     res.update(extras)
+    if args.copy_calibration:
+        nb = args.copy_calibration << 20
+        a = hip.DeviceArray((nb // 4,), np.float32)
+        b = hip.DeviceArray((nb // 4,), np.float32)
+        hip.check(hip._load().hipMemcpy(b.handle, a.handle, nb, hip.D2D))
+        hip.synchronize()
+        res['copy_calibration_bytes'] = nb
     print(json.dumps(res), flush=True)
 
 
