@@ -505,7 +505,7 @@ void stage_mark(fb_plan *P, hipStream_t s, const char *name)
     P->stage_names.push_back(name);
 }
 
-int plan_run(fb_plan *P, const double *pts, const double *val, float *out, hipStream_t s)
+int plan_run(fb_plan *P, const double *pts, const double *val, float *out, float *wsave, hipStream_t s)
 {
     int rc;
     for (hipEvent_t e : P->events) (void)hipEventDestroy(e);
@@ -547,7 +547,7 @@ int plan_run(fb_plan *P, const double *pts, const double *val, float *out, hipSt
             const int fin = a == P->dim - 1;
             if ((rc = launch_fast_pass(a == 0, P->d_fields, P->d_offsets, P->d_work[a], P->nwork[a], P->d_pass[a],
                                        kFastThreads, P->lds[a], ks, vs, P->d_line_start, P->d_line_count, pts, val,
-                                       P->d_v, P->d_w, out, fin, s)))
+                                       P->d_v, P->d_w, out, fin ? wsave : nullptr, fin, s)))
                 return rc;
             stage_mark(P, s, pn[a]);
         }
@@ -660,7 +660,7 @@ int fb_plan_create(fb_plan **plan, const fb_field *fields, int nfields, int prec
 int fb_plan_execute(fb_plan *P, const double *d_pts, const double *d_val, float *d_out, void *stream)
 {
     if (!P || !d_pts || !d_val || !d_out) return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument");
-    return plan_run(P, d_pts, d_val, d_out, (hipStream_t)stream);
+    return plan_run(P, d_pts, d_val, d_out, nullptr, (hipStream_t)stream);
 }
 
 int fb_plan_execute_graph(fb_plan *P, const double *d_pts, const double *d_val, float *d_out, void *stream)
@@ -674,7 +674,7 @@ int fb_plan_execute_graph(fb_plan *P, const double *d_pts, const double *d_val, 
         const int timing = P->timing;
         P->timing = 0;
         FB_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        int rc = plan_run(P, d_pts, d_val, d_out, s);
+        int rc = plan_run(P, d_pts, d_val, d_out, nullptr, s);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s, &g);
         P->timing = timing;
@@ -686,6 +686,37 @@ int fb_plan_execute_graph(fb_plan *P, const double *d_pts, const double *d_val, 
     }
     FB_HIP_CHECK(hipGraphLaunch(P->gexec, s));
     return FB_OK;
+}
+
+int fb_plan_forward(fb_plan *P, const double *d_pts, const double *d_val, float *d_out, float *d_wsave, void *stream)
+{
+    if (!P || !d_pts || !d_val || !d_out || !d_wsave) return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (P->precision != FB_PREC_FP32C)
+        return fail(FB_ERR_UNSUPPORTED, "the training forward and the adjoint need a plan of precision FP32C");
+    return plan_run(P, d_pts, d_val, d_out, d_wsave, (hipStream_t)stream);
+}
+
+int fb_plan_backward(fb_plan *P, const double *d_pts, const double *d_val, const float *d_out, const float *d_wsave,
+                     const float *d_gout, double *d_gval, double *d_gpts, void *stream)
+{
+    if (!P || !d_pts || !d_val || !d_out || !d_wsave || !d_gout || !d_gval)
+        return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (P->precision != FB_PREC_FP32C)
+        return fail(FB_ERR_UNSUPPORTED, "the training forward and the adjoint need a plan of precision FP32C");
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    if ((rc = launch_offsets(P->d_fields, P->nfields, d_val, P->d_offsets, s))) return rc;
+    if ((rc = launch_adj_prep(P->d_fields, P->d_cell_prefix, P->nfields, P->ncells, P->d_offsets, d_out, d_wsave,
+                              d_gout, P->d_v, P->d_w, s)))
+        return rc;
+    // the same scaled line passes, reading both fields from HBM on every axis
+    for (int a = 0; a < P->dim; ++a)
+        if ((rc = launch_fast_pass(false, P->d_fields, P->d_offsets, P->d_work[a], P->nwork[a], P->d_pass[a],
+                                   kFastThreads, P->lds[a], nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                   P->d_v, P->d_w, nullptr, nullptr, 0, s)))
+            return rc;
+    return launch_adj_gather(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints, d_pts, d_val, P->d_offsets,
+                             P->d_v, P->d_w, d_gval, d_gpts, s);
 }
 
 int fb_plan_set_timing(fb_plan *P, int enable)
@@ -820,7 +851,7 @@ int fb_barnes(int method, int precision, int dim, int64_t n, const double *pts, 
     if ((rc = plan_create(&P, &f, 1, precision))) return rc;
     float *d_out = (float *)out;
     if (!on_dev && (rc = sc.get(&d_out, cells))) { delete P; return rc; }
-    rc = plan_run(P, d_pts, d_val, d_out, s);
+    rc = plan_run(P, d_pts, d_val, d_out, nullptr, s);
     if (!rc && !on_dev) {
         hipError_t e = hipMemcpyAsync(out, d_out, cells * sizeof(float), hipMemcpyDeviceToHost, s);
         if (e != hipSuccess) rc = fail(FB_ERR_HIP, std::string("copy of the result failed: ") + hipGetErrorString(e));
@@ -905,7 +936,7 @@ int fb_accumulate_lines(int tail, int precision, int64_t nlines, int64_t len, in
         if ((rc = launch_f64_to_f32(d_io, d_v, cells, s))) return rc;
         if ((rc = launch_zero(d_w, cells * 4, s))) return rc;
         if ((rc = launch_fast_pass(false, d_f, d_off, d_work, (int)work.size(), d_pf, kFastThreads, lds, nullptr, nullptr,
-                                   nullptr, nullptr, nullptr, nullptr, d_v, d_w, nullptr, 0, s)))
+                                   nullptr, nullptr, nullptr, nullptr, d_v, d_w, nullptr, nullptr, 0, s)))
             return rc;
         if ((rc = launch_f32_to_f64(d_v, d_io, cells, 1.0, s))) return rc;
     }
@@ -996,7 +1027,7 @@ int fb_convolve_grid(int tail, int precision, int dim, const int64_t *size, cons
             FB_HIP_CHECK(hipMemcpy(d_work, work.data(), work.size() * sizeof(PassWork), hipMemcpyHostToDevice));
             FB_HIP_CHECK(hipMemcpy(d_pf, pf.data(), sizeof(PassField), hipMemcpyHostToDevice));
             if ((rc = launch_fast_pass(false, d_f, d_off, d_work, (int)work.size(), d_pf, kFastThreads, lds, nullptr,
-                                       nullptr, nullptr, nullptr, nullptr, nullptr, d_v, d_w, nullptr, 0, s)))
+                                       nullptr, nullptr, nullptr, nullptr, nullptr, d_v, d_w, nullptr, nullptr, 0, s)))
                 return rc;
             FB_HIP_CHECK(hipStreamSynchronize(s));   // host vectors above go out of scope
         }

@@ -490,7 +490,7 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     const PassField *__restrict__ passes, const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
     const uint32_t *__restrict__ line_start, const uint32_t *__restrict__ line_count, const double *__restrict__ pts,
     const double *__restrict__ val, float *__restrict__ gv, float *__restrict__ gw, float *__restrict__ out,
-    int final_pass)
+    float *__restrict__ wsave, int final_pass)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const PassWork wk = work[blockIdx.x];
@@ -625,6 +625,8 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
             }
         __syncthreads();
         tile_copy<false>(out + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
+        // training forward: keep the normalised weight field for the adjoint
+        if (wsave) tile_copy<false>(wsave + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
     } else {
         tile_copy<false>(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
         tile_copy<false>(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
@@ -728,6 +730,89 @@ __global__ __launch_bounds__(kBlock) void k_naive(const FieldDev *__restrict__ f
     }
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the FP32C batch (no counterpart in the reference, which has no gradient).
+//
+// On unmasked nodes f = V/W + offset with V = B S (val - offset), W = B S 1, where S is
+// the bilinear injection and B the product of the per-axis n-fold tailed box filters
+// (zero-padded banded symmetric matrices, so B is self-adjoint).  For g = dL/df:
+//   a = g/W,  b = -g V/W^2 = -a (f - offset),  A = B a,  Bb = B b
+//   dL/dval_k  = sum_c w_c(p_k) A[c]
+//   dL/dp_k,m  = sum_c (dw_c/dp_m) ((val_k - offset) A[c] + Bb[c])
+// The offset drops out (V/W does not depend on it).  The FP32C passes carry the
+// normalisation (1/(K+2 alpha))^n per axis in both V and W; applying the same scaled
+// operator to a = g/W_scaled gives the same A, so the gradient needs no rescaling.
+
+// a and b at every node of the batch in the cell layout; zero where the forward result
+// is NaN (masked) or the incoming gradient is NaN, and in the alignment gaps.
+__global__ __launch_bounds__(kBlock) void k_adj_prep(const FieldDev *__restrict__ fields,
+                                                     const int64_t *__restrict__ cell_prefix, int nfields,
+                                                     int64_t ncells, const double *__restrict__ offsets,
+                                                     const float *__restrict__ out, const float *__restrict__ wsave,
+                                                     const float *__restrict__ gout, float *__restrict__ av,
+                                                     float *__restrict__ bv)
+{
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= ncells) return;
+    const int f = find_seg(cell_prefix, nfields, g);
+    const FieldDev &F = fields[f];
+    const int64_t i = g - F.cell_off;
+    float ra = 0.f, rb = 0.f;
+    if (i < F.size[0] * F.size[1] * F.size[2]) {
+        const float fo = out[F.out_off + i];
+        const float go = gout[F.out_off + i];
+        if (fo == fo && go == go) {
+            const double a = (double)go / (double)wsave[F.out_off + i];
+            ra = (float)a;
+            rb = (float)(-a * ((double)fo - offsets[f]));
+        }
+    }
+    av[g] = ra;
+    bv[g] = rb;
+}
+
+// Transposed injection: one thread per point gathers A and Bb at its 2^dim corners.
+// Points dropped by the injection (outside the grid, NaN) get zero gradients.
+__global__ __launch_bounds__(kBlock) void k_adj_gather(const FieldDev *__restrict__ fields,
+                                                       const int64_t *__restrict__ pt_prefix, int nfields,
+                                                       int64_t npoints, const double *__restrict__ pts,
+                                                       const double *__restrict__ val,
+                                                       const double *__restrict__ offsets,
+                                                       const float *__restrict__ A, const float *__restrict__ Bb,
+                                                       double *__restrict__ gval, double *__restrict__ gpts)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= npoints) return;
+    const int f = find_seg(pt_prefix, nfields, i);
+    const FieldDev &F = fields[f];
+    const int dim = F.dim;
+    PointPos pp;
+    const bool inside = point_pos(F, pts + i * dim, pp);
+    double gv = 0.0, gp[3] = {0.0, 0.0, 0.0};
+    if (inside) {
+        const double vk = val[i] - offsets[f];
+        const int64_t W = F.size[0], H = F.size[1];
+        const int corners = 1 << dim;
+        for (int cb = 0; cb < corners; ++cb) {
+            const int bx = cb & 1, by = (cb >> 1) & 1, bz = (cb >> 2) & 1;
+            const int64_t cell = F.cell_off + ((pp.i[2] + bz) * H + (pp.i[1] + by)) * W + (pp.i[0] + bx);
+            const double a = (double)A[cell], b = (double)Bb[cell];
+            const double fx = bx ? pp.w[0] : 1.0 - pp.w[0];
+            const double fy = dim >= 2 ? (by ? pp.w[1] : 1.0 - pp.w[1]) : 1.0;
+            const double fz = dim >= 3 ? (bz ? pp.w[2] : 1.0 - pp.w[2]) : 1.0;
+            const double sx = bx ? 1.0 : -1.0, sy = by ? 1.0 : -1.0, sz = bz ? 1.0 : -1.0;
+            gv += corner_weight(dim, pp, bx, by, bz) * a;
+            const double t = vk * a + b;
+            gp[0] += sx * fy * fz * t;
+            gp[1] += fx * sy * fz * t;
+            gp[2] += fx * fy * sz * t;
+        }
+    }
+    gval[i] = gv;
+    if (gpts)
+        for (int m = 0; m < dim; ++m) gpts[i * dim + m] = gp[m] / F.step[m];
+}
+
 inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -822,8 +907,8 @@ int fast_pass_init()
 int launch_fast_pass(bool inject, const FieldDev *d_fields, const double *d_offsets, const PassWork *d_work,
                      int nwork, const PassField *d_pass, int threads, size_t lds_bytes, const uint32_t *d_keys,
                      const uint32_t *d_vals, const uint32_t *d_line_start, const uint32_t *d_line_count,
-                     const double *d_pts, const double *d_val, float *d_v, float *d_w, float *d_out, int final_pass,
-                     hipStream_t s)
+                     const double *d_pts, const double *d_val, float *d_v, float *d_w, float *d_out, float *d_wsave,
+                     int final_pass, hipStream_t s)
 {
     if (nwork <= 0) return FB_OK;
     if (threads != kFastBlock) {
@@ -833,12 +918,34 @@ int launch_fast_pass(bool inject, const FieldDev *d_fields, const double *d_offs
     if (inject) {
         hipLaunchKernelGGL(k_fast_pass<true>, dim3(nwork), dim3(threads), lds_bytes, s, d_fields, d_offsets, d_work,
                            d_pass, d_keys, d_vals, d_line_start, d_line_count, d_pts, d_val, d_v, d_w, d_out,
-                           final_pass);
+                           d_wsave, final_pass);
     } else {
         hipLaunchKernelGGL(k_fast_pass<false>, dim3(nwork), dim3(threads), lds_bytes, s, d_fields, d_offsets,
                            d_work, d_pass, d_keys, d_vals, d_line_start, d_line_count, d_pts, d_val, d_v, d_w, d_out,
-                           final_pass);
+                           d_wsave, final_pass);
     }
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
+int launch_adj_prep(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfields, int64_t ncells,
+                    const double *d_offsets, const float *d_out, const float *d_wsave, const float *d_gout,
+                    float *d_a, float *d_b, hipStream_t s)
+{
+    if (ncells <= 0) return FB_OK;
+    hipLaunchKernelGGL(k_adj_prep, dim3(nblocks(ncells)), dim3(kBlock), 0, s, d_fields, d_cell_prefix, nfields,
+                       ncells, d_offsets, d_out, d_wsave, d_gout, d_a, d_b);
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
+int launch_adj_gather(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
+                      const double *d_pts, const double *d_val, const double *d_offsets, const float *d_A,
+                      const float *d_Bb, double *d_gval, double *d_gpts, hipStream_t s)
+{
+    if (npoints <= 0) return FB_OK;
+    hipLaunchKernelGGL(k_adj_gather, dim3(nblocks(npoints)), dim3(kBlock), 0, s, d_fields, d_pt_prefix, nfields,
+                       npoints, d_pts, d_val, d_offsets, d_A, d_Bb, d_gval, d_gpts);
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }

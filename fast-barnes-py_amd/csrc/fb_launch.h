@@ -45,7 +45,15 @@ int launch_fast_pass(bool inject, const FieldDev *d_fields, const double *d_offs
                      int nwork, const PassField *d_pass, int threads, size_t lds_bytes, const uint32_t *d_keys,
                      const uint32_t *d_vals, const uint32_t *d_line_start, const uint32_t *d_line_count,
                      const double *d_pts, const double *d_val, float *d_v, float *d_w, float *d_out,
-                     int final_pass, hipStream_t stream);
+                     float *d_wsave, int final_pass, hipStream_t stream);
+
+// fb_kernels.hip -- adjoint of the FP32C batch
+int launch_adj_prep(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfields, int64_t ncells,
+                    const double *d_offsets, const float *d_out, const float *d_wsave, const float *d_gout,
+                    float *d_a, float *d_b, hipStream_t stream);
+int launch_adj_gather(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
+                      const double *d_pts, const double *d_val, const double *d_offsets, const float *d_A,
+                      const float *d_Bb, double *d_gval, double *d_gpts, hipStream_t stream);
 
 // fb_kernels.hip -- elementwise helpers
 int launch_zero(void *p, int64_t bytes, hipStream_t stream);   // bytes: multiple of 4

@@ -74,6 +74,8 @@ SIGNATURES = {
                                       ctypes.c_uint]),
     'fb_plan_execute': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     'fb_plan_execute_graph': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    'fb_plan_forward': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    'fb_plan_backward': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     'fb_plan_set_timing': (ctypes.c_int, [_vp, ctypes.c_int]),
     'fb_plan_stage_times': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_int]),
@@ -82,15 +84,40 @@ SIGNATURES = {
 }
 
 _lib = None
+# which HIP runtime instance the library is bound to: 'torch' or 'system'
+RUNTIME = None
+
+
+def _bind_runtime():
+    """One HIP runtime per process.  PyTorch-ROCm ships its own libamdhip64 (same soname as
+    the system ROCm one); two instances in one process do not share devices, streams or
+    allocations.  When PyTorch is importable, its runtime is loaded first, so that the
+    library's libamdhip64.so.7 dependency resolves to it and torch's streams and tensors
+    can be handed to the C ABI.  Otherwise the system ROCm runtime (the library's RUNPATH)
+    is used.  FASTBARNES_SYSTEM_HIP=1 forces the system runtime (no torch interop)."""
+    if os.environ.get('FASTBARNES_SYSTEM_HIP') == '1':
+        return 'system'
+    try:
+        import torch
+    except ImportError:
+        return 'system'
+    if not getattr(torch.version, 'hip', None):
+        return 'system'
+    path = os.path.join(os.path.dirname(torch.__file__), 'lib', 'libamdhip64.so')
+    if not os.path.exists(path):
+        return 'system'
+    ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+    return 'torch'
 
 
 def lib():
     """Loads libfastbarnes.so once; raises RuntimeError if it is not built."""
-    global _lib
+    global _lib, RUNTIME
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError('libfastbarnes.so is not built (expected at %s); run `make -C fast-barnes-py_amd` '
                                'or __graft_entry__.build()' % LIB_PATH)
+        RUNTIME = _bind_runtime()
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -106,6 +106,17 @@ class BarnesBatch:
         fn = lib().fb_plan_execute_graph if graph else lib().fb_plan_execute
         check(fn(self._plan, d_pts, d_val, d_out, stream))
 
+    def forward(self, d_pts, d_val, d_out, d_wsave, stream):
+        """Training forward (FP32C plans): execute() that also writes the normalised weight
+        field (float32, total_out elements) to d_wsave for backward()."""
+        check(lib().fb_plan_forward(self._plan, d_pts, d_val, d_out, d_wsave, stream))
+
+    def backward(self, d_pts, d_val, d_out, d_wsave, d_gout, d_gval, d_gpts, stream):
+        """Adjoint of forward(): dL/dout (float32, total_out) -> dL/dval (float64, one per
+        point) and, if d_gpts is not None/0, dL/dpts (float64, points x dim)."""
+        check(lib().fb_plan_backward(self._plan, d_pts, d_val, d_out, d_wsave, d_gout, d_gval, d_gpts or None,
+                                     stream))
+
     def set_timing(self, enable):
         check(lib().fb_plan_set_timing(self._plan, int(bool(enable))))
 

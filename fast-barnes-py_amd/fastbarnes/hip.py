@@ -4,7 +4,6 @@ events for callers that do not use PyTorch.  Only plumbing: every computation ru
 in libfastbarnes.so.
 """
 import ctypes
-import os
 
 import numpy as np
 
@@ -18,17 +17,13 @@ D2D = 3
 def _load():
     global _hip
     if _hip is None:
-        cands = [os.path.join(os.environ.get('ROCM_PATH', '/opt/rocm'), 'lib', 'libamdhip64.so'),
-                 'libamdhip64.so', 'libamdhip64.so.7']
-        err = None
-        for c in cands:
-            try:
-                _hip = ctypes.CDLL(c)
-                break
-            except OSError as e:
-                err = e
-        if _hip is None:
-            raise RuntimeError('cannot load the HIP runtime: %s' % err)
+        # the runtime instance libfastbarnes.so is bound to (resolved by soname)
+        from fastbarnes import _lib
+        _lib.lib()
+        try:
+            _hip = ctypes.CDLL('libamdhip64.so.7')
+        except OSError as e:
+            raise RuntimeError('cannot load the HIP runtime: %s' % e)
         vp = ctypes.c_void_p
         _hip.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
         _hip.hipFree.argtypes = [vp]

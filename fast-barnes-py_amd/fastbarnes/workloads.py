@@ -32,8 +32,9 @@ def values_cfg(pts, seed):
     return 15.0 + 10.0 * np.sin(pts[:, 0] / 8.0) * np.cos(pts[:, 1] / 6.0) + rng.normal(0.0, 0.5, len(pts))
 
 
-def values_sinusoids(pts, seed, nwaves=8):
-    """Random smooth field: sum of 8 sinusoids with random phases, plus N(0, 0.5)."""
+def smooth_field(pts, seed, nwaves=8):
+    """Random smooth field at the points: 10 + sum of 8 sinusoids with random wave vectors,
+    amplitudes and phases."""
     rng = np.random.default_rng(seed + 104729)
     v = np.zeros(len(pts))
     for _ in range(nwaves):
@@ -41,7 +42,13 @@ def values_sinusoids(pts, seed, nwaves=8):
         a = rng.uniform(0.5, 3.0)
         phi = rng.uniform(0.0, 2 * np.pi)
         v += a * np.sin(pts @ k + phi)
-    return 10.0 + v + rng.normal(0.0, 0.5, len(pts))
+    return 10.0 + v
+
+
+def values_sinusoids(pts, seed, nwaves=8):
+    """Random smooth field (smooth_field) plus N(0, 0.5) noise."""
+    rng = np.random.default_rng(seed + 7)
+    return smooth_field(pts, seed, nwaves) + rng.normal(0.0, 0.5, len(pts))
 
 
 def config1(seed=1):
@@ -78,3 +85,20 @@ def config4(seed=4, n=200000, size=(512, 512, 128)):
     pts = rng.uniform(0.0, 1.0, (n, 3)) * ext
     val = np.sin(pts[:, 0] / 5) + np.cos(pts[:, 1] / 7) + pts[:, 2] / 16 + rng.normal(0.0, 0.1, n)
     return dict(pts=pts, val=val, sigma=1.5, x0=np.zeros(3), step=0.125, size=size, num_iter=4, max_dist=5.0)
+
+
+def config5(seed=5, n=100000, size=(2400, 1200)):
+    """Differentiable 2D workload: N=100000 uniform+cluster stations, grid 2400x1200, step 75/W,
+    sigma=1, n=4, values as in config1; plus an independently seeded smooth target field on
+    the grid (shape size[::-1]) for the loss 0.5*||f - target||^2."""
+    W, H = size
+    pts = stations_2d(n, seed)
+    step = 75.0 / W
+    x0 = np.array([-26.0, 34.0])
+    xs = x0[0] + step * np.arange(W)
+    ys = x0[1] + step * np.arange(H)
+    gx, gy = np.meshgrid(xs, ys)
+    tp = np.column_stack([gx.ravel(), gy.ravel()])
+    target = smooth_field(tp, seed + 31337).reshape(H, W)
+    return dict(pts=pts, val=values_cfg(pts, seed), sigma=1.0, x0=x0, step=step, size=(W, H), num_iter=4,
+                max_dist=3.5, target=target)

@@ -140,6 +140,22 @@ int fb_plan_execute(fb_plan *plan, const double *d_pts, const double *d_val, flo
  * call for a (pts, val, out, stream) tuple captures, later calls replay. */
 int fb_plan_execute_graph(fb_plan *plan, const double *d_pts, const double *d_val, float *d_out, void *stream);
 
+/* Training forward: fb_plan_execute that also writes the normalised weight field of every
+ * output node (float32, laid out like d_out) to d_wsave, for fb_plan_backward.
+ * FP32C plans only.  No counterpart in the reference (it has no gradient); it serves
+ * callers that differentiate through the interpolation. */
+int fb_plan_forward(fb_plan *plan, const double *d_pts, const double *d_val, float *d_out, float *d_wsave,
+                    void *stream);
+
+/* Adjoint of fb_plan_forward.  Given d_gout = dL/d(out) (float32, laid out like d_out;
+ * nodes where out is NaN are ignored), writes dL/d(val) to d_gval (one double per point)
+ * and, when d_gpts is not NULL, dL/d(pts) to d_gpts (n x dim doubles).  Points the
+ * injection drops (outside the grid) get zero gradients.  pts, val, out and wsave must be
+ * those of the forward call.  Uses the plan's workspace: calls on one plan are
+ * stream-ordered, not concurrent.  No counterpart in the reference. */
+int fb_plan_backward(fb_plan *plan, const double *d_pts, const double *d_val, const float *d_out,
+                     const float *d_wsave, const float *d_gout, double *d_gval, double *d_gpts, void *stream);
+
 /* Per-stage device time (ms) of the last fb_plan_execute run with timing enabled,
  * measured with HIP events recorded on the execution stream.  names receives a
  * comma-separated stage list.  enable != 0 turns event recording on. */

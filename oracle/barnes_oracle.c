@@ -432,3 +432,131 @@ void orc_interpolate_radius_bf(const double *pts, const double *val_in, int64_t 
     }
     free(val);
 }
+
+/* ---- adjoint (no reference counterpart: the reference has no gradient) ----
+ *
+ * Derived from the forward path above (:328-367).  On unmasked nodes the forward value
+ * is f = V/W + offset with V = B S (val - offset) and W = B S 1: S the bilinear
+ * injection (orc_inject), B the per-axis n-fold line filters of orc_convolve.  Each
+ * line filter is a banded symmetric matrix (the same weights left and right of the
+ * centre, zero padding), so B is self-adjoint.  For g = dL/df:
+ *   a = g/W, b = -g V/W^2 (zero on masked nodes), A = B a, Bb = B b
+ *   dL/dval_k  = sum_c w_c A[c]
+ *   dL/dpts_km = sum_c (dw_c/dp_m) ((val_k - offset) A[c] + Bb[c])
+ * where dw_c/dp_m differentiates the corner weight product in axis m and divides by
+ * step[m].  Points dropped by the injection get zero gradients. */
+static void filter_grid(int tail, int dim, double *grid, const int64_t *size, const int32_t *kernel_size,
+                        int64_t num_iter, const double *alpha)
+{
+    int64_t W = size[0], H = dim > 1 ? size[1] : 1, D = dim > 2 ? size[2] : 1;
+    int64_t maxlen = W > H ? W : H; if (D > maxlen) maxlen = D;
+    double *a = (double *)malloc(sizeof(double) * maxlen);
+    double *b = (double *)malloc(sizeof(double) * maxlen);
+    int64_t nmax = W * H > W * D ? W * H : W * D; if (H * D > nmax) nmax = H * D;
+    int64_t *bases = (int64_t *)calloc(nmax > 0 ? nmax : 1, sizeof(int64_t));
+    int64_t nl = 0;
+    for (int64_t z = 0; z < D; z++) for (int64_t y = 0; y < H; y++) bases[nl++] = (z * H + y) * W;
+    convolve_axis(grid, nl, bases, W, 1, kernel_size[0], num_iter, tail, alpha[0], a, b);
+    if (dim >= 2) {
+        nl = 0;
+        for (int64_t z = 0; z < D; z++) for (int64_t x = 0; x < W; x++) bases[nl++] = z * H * W + x;
+        convolve_axis(grid, nl, bases, H, W, kernel_size[1], num_iter, tail, alpha[1], a, b);
+    }
+    if (dim >= 3) {
+        nl = 0;
+        for (int64_t y = 0; y < H; y++) for (int64_t x = 0; x < W; x++) bases[nl++] = y * W + x;
+        convolve_axis(grid, nl, bases, D, W * H, kernel_size[2], num_iter, tail, alpha[2], a, b);
+    }
+    free(a); free(b); free(bases);
+}
+
+/* out: the filtered copy of grid_in (B x); exposes the self-adjoint operator for tests */
+void orc_filter_grid(int tail, int dim, const double *grid_in, double *out, const double *sigma, const double *step,
+                     const int64_t *size, int64_t num_iter)
+{
+    int32_t T[3], K[3];
+    double alpha[3] = {0, 0, 0};
+    if (tail) {
+        orc_half_kernel_size_opt(dim, sigma, step, num_iter, T);
+        orc_tail_value(dim, sigma, step, num_iter, alpha);
+    } else {
+        orc_half_kernel_size(dim, sigma, step, num_iter, T);
+    }
+    for (int m = 0; m < dim; m++) K[m] = 2 * T[m] + 1;
+    int64_t total = 1;
+    for (int m = 0; m < dim; m++) total *= size[m];
+    memcpy(out, grid_in, sizeof(double) * total);
+    filter_grid(tail, dim, out, size, K, num_iter, alpha);
+}
+
+void orc_adjoint(int tail, int dim, const double *pts, const double *val_in, int64_t n, const double *sigma,
+                 const double *x0, const double *step, const int64_t *size, int64_t num_iter,
+                 double max_dist_weight, const double *g, double *gval, double *gpts)
+{
+    int64_t total = 1;
+    for (int m = 0; m < dim; m++) total *= size[m];
+    float *fout = (float *)malloc(sizeof(float) * total);
+    double *vg = (double *)malloc(sizeof(double) * total);
+    double *wg = (double *)malloc(sizeof(double) * total);
+    orc_interpolate_convol(tail, dim, pts, val_in, n, sigma, x0, step, size, num_iter, max_dist_weight, fout, vg, wg);
+    double *val = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    memcpy(val, val_in, sizeof(double) * n);
+    orc_normalize_values(val, n);
+    /* wg is NaN on masked nodes (orc_convolve), so the comparison drops them */
+    for (int64_t i = 0; i < total; i++) {
+        if (wg[i] == wg[i] && g[i] == g[i]) {
+            double a = g[i] / wg[i];
+            vg[i] = -a * (vg[i] / wg[i]);
+            wg[i] = a;
+        } else {
+            vg[i] = 0.0;
+            wg[i] = 0.0;
+        }
+    }
+    double *A = wg, *Bb = vg;
+    double sg[3], st[3];
+    for (int m = 0; m < dim; m++) { sg[m] = sigma[m]; st[m] = step[m]; }
+    double *tmp = (double *)malloc(sizeof(double) * total);
+    orc_filter_grid(tail, dim, A, tmp, sg, st, size, num_iter);
+    memcpy(A, tmp, sizeof(double) * total);
+    orc_filter_grid(tail, dim, Bb, tmp, sg, st, size, num_iter);
+    memcpy(Bb, tmp, sizeof(double) * total);
+    const int64_t W = size[0], H = dim > 1 ? size[1] : 1;
+    for (int64_t k = 0; k < n; k++) {
+        double c[3], w[3];
+        int64_t ci[3];
+        int inside = 1;
+        for (int m = 0; m < dim; m++) {
+            c[m] = (pts[k * dim + m] - x0[m]) / step[m];
+            if (!(c[m] >= 0.0) || c[m] >= (double)(size[m] - 1)) inside = 0;
+        }
+        double gv = 0.0, gp[3] = {0.0, 0.0, 0.0};
+        if (inside) {
+            for (int m = 0; m < dim; m++) { ci[m] = (int64_t)c[m]; w[m] = c[m] - (double)ci[m]; }
+            for (int cb = 0; cb < (1 << dim); cb++) {
+                int64_t idx = 0, mul = 1;
+                double f[3], d[3];
+                for (int m = 0; m < dim; m++) {
+                    int bit = (cb >> m) & 1;
+                    idx += (ci[m] + bit) * mul;
+                    mul *= size[m];
+                    f[m] = bit ? w[m] : 1.0 - w[m];
+                    d[m] = bit ? 1.0 : -1.0;
+                }
+                double wc = 1.0;
+                for (int m = 0; m < dim; m++) wc *= f[m];
+                gv += wc * A[idx];
+                double t = val[k] * A[idx] + Bb[idx];
+                for (int m = 0; m < dim; m++) {
+                    double dm = d[m];
+                    for (int q = 0; q < dim; q++) if (q != m) dm *= f[q];
+                    gp[m] += dm * t;
+                }
+            }
+        }
+        gval[k] = gv;
+        if (gpts) for (int m = 0; m < dim; m++) gpts[k * dim + m] = gp[m] / step[m];
+    }
+    (void)W; (void)H;
+    free(tmp); free(val); free(vg); free(wg); free(fout);
+}

@@ -62,6 +62,9 @@ def lib():
         L.orc_interpolate_naive.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp, _dp, _dp, _i64p, _dp]
         L.orc_interpolate_radius_bf.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_double, _dp, _dp, _i64p,
                                                 ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_filter_grid.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _i64p, ctypes.c_int64]
+        L.orc_adjoint.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp, _dp, _dp, _i64p,
+                                  ctypes.c_int64, ctypes.c_double, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -208,3 +211,29 @@ def inject(pts, val, x0, step, size):
     lib().orc_inject(dim, _d(vg), _d(wg), _d(pts), _d(v), len(v), _d(_vec(x0, dim)), _d(_vec(step, dim)),
                      sz.ctypes.data_as(_i64p))
     return vg, wg, off
+
+
+def filter_grid(grid, sigma, step, num_iter=4, tail=True):
+    """B x: the unscaled per-axis line filters of the convolution applied to a grid
+    (shape size[::-1]); the operator the adjoint transposes."""
+    g = np.ascontiguousarray(np.asarray(grid, dtype=np.float64))
+    dim = g.ndim
+    sz = np.asarray(g.shape[::-1], dtype=np.int64)
+    out = np.empty_like(g)
+    lib().orc_filter_grid(int(tail), dim, _d(g), _d(out), _d(_vec(sigma, dim)), _d(_vec(step, dim)),
+                          sz.ctypes.data_as(_i64p), num_iter)
+    return out
+
+
+def adjoint(pts, val, sigma, x0, step, size, g, num_iter=4, max_dist=3.5, tail=True):
+    """FP64 adjoint of interpolate_convol: for g = dL/df (grid, shape size[::-1]) returns
+    (dL/dval (n,), dL/dpts (n, dim)).  No reference counterpart; derived from the forward."""
+    pts, dim, val, sg, x0v, st, sz = _common(pts, val, sigma, x0, step, size)
+    mdw = math.exp(-max_dist ** 2 / 2)
+    gg = np.ascontiguousarray(np.asarray(g, dtype=np.float64))
+    assert gg.shape == tuple(int(s) for s in sz[::-1])
+    gval = np.empty(len(val))
+    gpts = np.empty((len(val), dim))
+    lib().orc_adjoint(int(tail), dim, _d(pts), _d(val), len(val), _d(sg), _d(x0v), _d(st),
+                      sz.ctypes.data_as(_i64p), num_iter, mdw, _d(gg), _d(gval), _d(gpts))
+    return gval, gpts
