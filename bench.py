@@ -11,9 +11,13 @@ before the timed region; each step replays the batch's captured hipGraph.
 Printed JSON line: value = grid points per second of the whole batch; roofline = the
 dominant kernel's algorithmic bytes over its average duration (HIP events on its
 stream); cpu_baseline = the CPU oracle (FP64 C restatement of the reference path,
-one core) on a bounded sample of the same fields.
+one core) on a bounded sample of the same fields.  'workloads' adds the other
+workloads, each with its throughput, pipeline roofline fraction and error against the
+FP64 path: cfg3 (256 uneven 2D fields), cfg4 (3D 512x512x128) and cfg4_batch (32 3D
+fields of 256x256x64), cfg5 (16 fields of 2400x1200, N=100000, forward + adjoint).
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--fields 64] [--cpu-seconds 10]
+                    [--legs cfg3,cfg4,cfg5|none] [--leg-steps 10]
 """
 import argparse
 import json
@@ -70,6 +74,149 @@ def cpu_baseline(cfgs, seconds):
                 sample='%d field(s) of 2400x1200, N=3490, n=4 (FP64 oracle, -O2, 1 thread), %.2f s' % (done, dt))
 
 
+# algorithmic bytes per grid point / per sample point of the other workloads' pipelines
+PIPE_BYTES = {2: (28.0, 24.0), 3: (52.0, 32.0)}
+# training (2D): forward also keeps the weight field (+8 B/pt); backward reads g, f, W (16),
+# round-trips two float32 fields (8 + 8), reads pts/val and writes the gradients (24 + 24)
+TRAIN_BYTES = (36.0 + 32.0, 24.0 + 48.0)
+
+
+def _descs(cfgs):
+    return [dict(sigma=c['sigma'], x0=c['x0'], step=c['step'], size=c['size'], n=len(c['val']),
+                 num_iter=c['num_iter'], max_dist=c['max_dist']) for c in cfgs]
+
+
+def _time_steps(fn, stream, steps, warmup):
+    from fastbarnes import hip
+    for _ in range(warmup):
+        fn()
+    stream.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_ms(e1) / steps
+
+
+def _errors(out, ref):
+    ok = ~np.isnan(ref) & ~np.isnan(out)
+    d = out[ok].astype(np.float64) - ref[ok].astype(np.float64)
+    return dict(rmse=float(np.sqrt(np.mean(d * d))), maxabs=float(np.max(np.abs(d))),
+                mask_mismatch=int((np.isnan(ref) != np.isnan(out)).sum()))
+
+
+def leg_batch(name, cfgs, steps, warmup, check_fields, note):
+    """FP32C batch of uneven fields (graph replays) and its error against the FP64 path
+    (the reference's operation order) on the first check_fields fields."""
+    from fastbarnes import hip
+    from fastbarnes.batched import BarnesBatch
+    log('%s: %d fields' % (name, len(cfgs)))
+    descs = _descs(cfgs)
+    batch = BarnesBatch(descs, precision='fp32c')
+    pts = np.concatenate([c['pts'] for c in cfgs])
+    val = np.concatenate([c['val'] for c in cfgs])
+    d_pts, d_val = hip.DeviceArray.from_numpy(pts), hip.DeviceArray.from_numpy(val)
+    d_out = hip.DeviceArray((batch.total_out,), np.float32)
+    s = hip.Stream()
+    ms = _time_steps(lambda: batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr, graph=True), s, steps, warmup)
+    batch.set_timing(True)
+    acc = {}
+    for _ in range(max(3, steps // 4)):
+        batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr)
+        for k, v in batch.stage_times().items():
+            acc.setdefault(k, []).append(v)
+    batch.set_timing(False)
+    npts, nsamp = batch.total_out, batch.total_points
+    bp, bs = PIPE_BYTES[batch.dim]
+    pipe = bp * npts + bs * nsamp
+    res = dict(workload=note, fields=len(cfgs), grid_points=npts, sample_points=nsamp,
+               value=round(npts / (ms * 1e-3), 1), unit='grid-pts/s', ms_per_step=round(ms, 4),
+               pipeline_alg_bytes=pipe, pipeline_frac=round(pipe / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+               stage_ms={k: round(float(np.mean(v)), 5) for k, v in acc.items()})
+    out = d_out.numpy()
+    b64 = BarnesBatch(descs[:check_fields], precision='fp64')
+    d_o64 = hip.DeviceArray((b64.total_out,), np.float32)
+    b64.execute(d_pts.ptr, d_val.ptr, d_o64.ptr, s.ptr)
+    s.synchronize()
+    n = b64.total_out
+    res['check_fields'] = check_fields
+    res.update({k + '_vs_fp64': v for k, v in _errors(out[:n], d_o64.numpy()).items()})
+    return res
+
+
+def leg_train(cfgs, steps, warmup):
+    """Config-5 training step: FP32C forward (keeping the weight field) + adjoint of a batch."""
+    from fastbarnes import hip
+    from fastbarnes.batched import BarnesBatch
+    from oracle import oracle
+    log('train: %d fields forward+backward' % len(cfgs))
+    descs = _descs(cfgs)
+    batch = BarnesBatch(descs, precision='fp32c')
+    pts = np.concatenate([c['pts'] for c in cfgs])
+    val = np.concatenate([c['val'] for c in cfgs])
+    tgt = np.concatenate([c['target'].ravel() for c in cfgs]).astype(np.float32)
+    d_pts, d_val = hip.DeviceArray.from_numpy(pts), hip.DeviceArray.from_numpy(val)
+    d_out = hip.DeviceArray((batch.total_out,), np.float32)
+    d_ws = hip.DeviceArray((batch.total_out,), np.float32)
+    d_gval = hip.DeviceArray((batch.total_points,), np.float64)
+    d_gpts = hip.DeviceArray(pts.shape, np.float64)
+    s = hip.Stream()
+    batch.forward(d_pts.ptr, d_val.ptr, d_out.ptr, d_ws.ptr, s.ptr)
+    s.synchronize()
+    out = d_out.numpy()
+    g = np.where(np.isnan(out), 0.0, out - tgt).astype(np.float32)   # dL/df of 0.5*||f - target||^2
+    d_g = hip.DeviceArray.from_numpy(g)
+
+    def step():
+        batch.forward(d_pts.ptr, d_val.ptr, d_out.ptr, d_ws.ptr, s.ptr)
+        batch.backward(d_pts.ptr, d_val.ptr, d_out.ptr, d_ws.ptr, d_g.ptr, d_gval.ptr, d_gpts.ptr, s.ptr)
+
+    ms = _time_steps(step, s, steps, warmup)
+    npts, nsamp = batch.total_out, batch.total_points
+    pipe = TRAIN_BYTES[0] * npts + TRAIN_BYTES[1] * nsamp
+    res = dict(workload='%d fields 2400x1200, N=%d each, sigma=1, n=4: fp32c forward + adjoint (dL/dval, dL/dpts) '
+                        'of 0.5*||f - target||^2' % (len(cfgs), len(cfgs[0]['val'])),
+               fields=len(cfgs), grid_points=npts, value=round(npts / (ms * 1e-3), 1), unit='grid-pts/s (fwd+bwd)',
+               ms_per_step=round(ms, 4), pipeline_alg_bytes=pipe,
+               pipeline_frac=round(pipe / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+    # gradient of field 0 against the FP64 adjoint
+    c = cfgs[0]
+    n0 = len(c['val'])
+    sh = tuple(c['size'][::-1])
+    o0 = out[:sh[0] * sh[1]]
+    g0 = np.where(np.isnan(o0), np.nan, o0 - tgt[:o0.size]).reshape(sh)
+    rval, rpts = oracle.adjoint(c['pts'], c['val'], c['sigma'], c['x0'], c['step'], c['size'], g0)
+    gv, gp = d_gval.numpy()[:n0], d_gpts.numpy()[:n0]
+    res['grad_val_rel_l2_vs_fp64'] = float(np.linalg.norm(gv - rval) / np.linalg.norm(rval))
+    res['grad_pts_rel_l2_vs_fp64'] = float(np.linalg.norm(gp - rpts) / np.linalg.norm(rpts))
+    return res
+
+
+def run_legs(names, args):
+    from fastbarnes import workloads
+    legs = {}
+    if 'cfg3' in names:
+        cfgs = [workloads.config3_field(i) for i in range(256)]
+        legs['cfg3'] = leg_batch('cfg3', cfgs, args.leg_steps, 2, 4,
+                                 '256 uneven fields: N~logU[500,50000], grid in {600x300,1200x600,2400x1200}, '
+                                 'step 75/W, sigma~U[0.5,2], num_iter~U{3..6}; fp32c, hipGraph replay')
+    if 'cfg4' in names:
+        legs['cfg4'] = leg_batch('cfg4', [workloads.config4()], args.leg_steps, 2, 1,
+                                 '3D 512x512x128, N=200000, step 0.125, sigma=1.5, n=4, max_dist=5; fp32c')
+        rng = np.random.default_rng(44)
+        cfgs = [workloads.config4(seed=400 + i, n=int(round(np.exp(rng.uniform(np.log(1e4), np.log(2e5))))),
+                                  size=(256, 256, 64)) for i in range(32)]
+        legs['cfg4_batch'] = leg_batch('cfg4_batch', cfgs, args.leg_steps, 2, 2,
+                                       '32 fields 3D 256x256x64, N~logU[1e4,2e5], step 0.125, sigma=1.5, n=4; '
+                                       'fp32c, hipGraph replay')
+    if 'cfg5' in names:
+        cfgs = [workloads.config5(seed=500 + i) for i in range(16)]
+        legs['cfg5'] = leg_train(cfgs, args.leg_steps, 2)
+    return legs
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -79,6 +226,9 @@ def main():
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-extras', action='store_true', help='skip accuracy and FP64-path legs')
+    ap.add_argument('--legs', default='cfg3,cfg4,cfg5',
+                    help="comma list of extra workloads reported under 'workloads' (cfg3, cfg4, cfg5) or 'none'")
+    ap.add_argument('--leg-steps', type=int, default=10)
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='after the run, one device-to-device copy of MIB MiB (a plain copy kernel of known '
                          'size that scales HBM byte counters)')
@@ -199,6 +349,8 @@ def main():
             speedup_vs_fp64_dropin=round(field_rate * fp64_dropin_s, 1),
             speedup_vs_fp64_device=round(field_rate * fp64_dev_ms * 1e-3, 1))
 
+    legs = run_legs([x for x in args.legs.split(',') if x and x != 'none'], args)
+
     cpu = None
     if not args.no_cpu:
         log('cpu baseline (%.0f s budget)' % args.cpu_seconds)
@@ -217,6 +369,8 @@ def main():
         'wall_s_timed': round(wall, 4),
     }
     res.update(extras)
+    if legs:
+        res['workloads'] = legs
     if args.copy_calibration:
         nb = args.copy_calibration << 20
         a = hip.DeviceArray((nb // 4,), np.float32)
