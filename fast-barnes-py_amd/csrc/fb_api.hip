@@ -191,7 +191,7 @@ hipStream_t lib_stream(int device)
 }
 
 // lanes per line / lines per workgroup of a fast line pass
-int pass_geometry(int64_t L, int T, int &lpl, int &lines, size_t &lds)
+int pass_geometry(int64_t L, int T, int &lpl, int &lines, int &rpad, size_t &lds)
 {
     const int64_t need = (L + kSegLen - 1) / kSegLen;
     lpl = 1;
@@ -199,14 +199,32 @@ int pass_geometry(int64_t L, int T, int &lpl, int &lines, size_t &lds)
     if (lpl > kBlock)
         return fail(FB_ERR_UNSUPPORTED, "line length " + std::to_string(L) + " exceeds the FP32C limit of " +
                                             std::to_string(kBlock * kSegLen));
-    lines = kBlock / lpl;
-    const size_t per_line = (size_t)(L + 2 * (int64_t)T + 2) * 2 * sizeof(float);
     // branch-free slides read up to kSegLen-1 floats past the last w line: reserve 64 floats
     const size_t reserve = 64 * sizeof(float);
-    while (lines > 1 && lines * per_line + reserve > kLdsBudget) {
-        lines >>= 1;
-        lpl <<= 1;
+    const int lane_lines = kBlock / lpl;
+    auto fit = [&](int rp) {
+        const size_t per_line = (size_t)(L + (int64_t)T + 1 + rp) * 2 * sizeof(float);
+        int64_t n = (int64_t)((kLdsBudget - reserve) / per_line);
+        n = std::min<int64_t>(n, lane_lines);
+        if (n >= 16) n &= ~(int64_t)3;   // float4 column tiles, at a loss below 1/4
+        return (int)std::max<int64_t>(n, 1);
+    };
+    // Zeros after each line: the reference's T+1.  A line of several segments may instead
+    // widen them to the overflow of its last segment (kSegLen - L mod kSegLen), so that the
+    // lane whose segment crosses the line end stores all of it and the pass re-zeroes the
+    // padding, instead of predicating every store.  The wider padding is taken when it does
+    // not cost lines per workgroup.
+    const int over = (int)((kSegLen - L % kSegLen) % kSegLen);
+    rpad = T + 1;
+    lines = fit(rpad);
+    if (need > 1 && over > T + 1) {
+        const int wide = fit(over);
+        if (wide >= lines) {
+            rpad = over;
+            lines = wide;
+        }
     }
+    const size_t per_line = (size_t)(L + (int64_t)T + 1 + rpad) * 2 * sizeof(float);
     lds = round_bytes(lines * per_line + reserve);
     if (lds > kLdsMax)
         return fail(FB_ERR_UNSUPPORTED, "line of length " + std::to_string(L) + " with half kernel " +
@@ -362,15 +380,17 @@ int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<Pa
         const FieldDev &F = P->fh[f];
         const int64_t W = F.size[0], H = F.size[1], D = F.size[2];
         const int64_t L = F.size[axis];
-        int lpl = 1, lines = 1;
+        int lpl = 1, lines = 1, rpad = 0;
         size_t l = 0;
-        int rc = pass_geometry(L, F.T[axis], lpl, lines, l);
+        int rc = pass_geometry(L, F.T[axis], lpl, lines, rpad, l);
         if (rc) return rc;
         lds = std::max(lds, l);
         PassField &p = pf[f];
         p.len = L;
         p.T = F.T[axis];
         p.lpl = lpl;
+        p.lines = lines;
+        p.rpad = rpad;
         p.alpha = F.alpha[axis];
         p.pscale = F.pscale[axis];
         std::vector<PassWork> items;
@@ -915,15 +935,17 @@ int fb_accumulate_lines(int tail, int precision, int64_t nlines, int64_t len, in
         PassWork *d_work;
         PassField *d_pf;
         if ((rc = sc.get(&d_v, cells)) || (rc = sc.get(&d_w, cells)) || (rc = sc.get(&d_off, 1))) return rc;
-        int lpl, lines;
+        int lpl, lines, rpad;
         size_t lds;
-        if ((rc = pass_geometry(len, F.T[0], lpl, lines, lds))) return rc;
-        PassField pf;
+        if ((rc = pass_geometry(len, F.T[0], lpl, lines, rpad, lds))) return rc;
+        PassField pf{};
         pf.len = len;
         pf.line_stride = len;
         pf.elem_stride = 1;
         pf.T = F.T[0];
         pf.lpl = lpl;
+        pf.lines = lines;
+        pf.rpad = rpad;
         pf.alpha = F.alpha[0];
         pf.pscale = 1.0;
         std::vector<PassWork> work;

@@ -51,7 +51,9 @@ struct PassField {
     int64_t line_stride;   // offset between consecutive lines of a workgroup
     int64_t elem_stride;   // offset between consecutive elements of a line
     int32_t T;
-    int32_t lpl;           // lanes per line (power of two, <= 64)
+    int32_t lpl;           // lanes per line (power of two)
+    int32_t rpad;          // zeros after each line in LDS (>= T+1)
+    int32_t lines;         // line capacity of a workgroup (lines * lpl <= 256)
     double alpha;
     double pscale;         // applied to the pass result
 };

@@ -383,9 +383,12 @@ __device__ __forceinline__ void slide1(const float *__restrict__ lin, int s, int
     }
 }
 
-__device__ __forceinline__ void stage_store(float *__restrict__ lin, int s, int n, const float (&sb)[kSegArr])
+// n: elements of the segment inside the line; full: the line's right padding holds at
+// least kSegArr-1 floats, so the whole segment may be stored (the caller re-zeroes the padding)
+__device__ __forceinline__ void stage_store(float *__restrict__ lin, int s, int n, bool full,
+                                            const float (&sb)[kSegArr])
 {
-    if (n == kSegArr) {
+    if (full || n == kSegArr) {
 #pragma unroll
         for (int j = 0; j < kSegArr; ++j) lin[s + j] = sb[j];
     } else {
@@ -504,9 +507,10 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     const int L = (int)P.len;
     const int T = P.T;
     const int lpl = P.lpl;
-    const int lines_cap = half / lpl;
+    const int lines_cap = P.lines;
     const int pad = T + 1;
-    const int lstride = L + 2 * pad;
+    const int rpad = P.rpad;
+    const int lstride = L + pad + rpad;
     const int nl = wk.nlines;
     float *sv = reinterpret_cast<float *>(smem);
     float *sw = sv + (size_t)lines_cap * lstride;
@@ -535,8 +539,9 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
         for (int i = tid; i < nz; i += nthr) z[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         for (int i = (nz << 2) + tid; i < tot; i += nthr) sv[i] = 0.f;
     } else {
-        for (int i = tid; i < lines_cap * 2 * pad; i += nthr) {
-            const int l = i / (2 * pad), r = i - l * 2 * pad;
+        const int npad = pad + rpad;
+        for (int i = tid; i < lines_cap * npad; i += nthr) {
+            const int l = i / npad, r = i - l * npad;
             const int o = l * lstride + (r < pad ? r : L + r);
             sv[o] = 0.f;
             sw[o] = 0.f;
@@ -596,17 +601,26 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     float *ln = mine + (size_t)(line_ok ? my_line : 0) * lstride + pad;
     const double alpha = P.alpha;
     const int niter = F.num_iter;
+    // whole-segment stores (when the host widened the right padding to the overflow of the
+    // last segment): the segment crossing the line end writes `over` floats into the
+    // padding; the line's lanes then re-zero those of them the next iteration reads (within
+    // the first T+1), after the stores in the wave's program order
+    const int over = (kSegArr - L % kSegArr) % kSegArr;
+    const bool full = L > kSegArr && rpad >= over;
+    const int nclr = full ? min(pad, over) : 0;
     float sb[kSegArr];
-    for (int it = 0; it + 1 < niter; ++it) {
-        if (act) slide1<false>(ln, s0, T, alpha, 1.0, sb);
+    for (int it = 0; it < niter; ++it) {
+        const bool last = it + 1 == niter;
+        if (act) {
+            if (last) slide1<true>(ln, s0, T, alpha, P.pscale, sb);
+            else slide1<false>(ln, s0, T, alpha, 1.0, sb);
+        }
         __syncthreads();
-        if (act) stage_store(ln, s0, myseg, sb);
+        if (act) stage_store(ln, s0, myseg, full, sb);
+        if (line_ok)
+            for (int e = sid; e < nclr; e += lpl) ln[L + e] = 0.f;
         __syncthreads();
     }
-    if (act) slide1<true>(ln, s0, T, alpha, P.pscale, sb);
-    __syncthreads();
-    if (act) stage_store(ln, s0, myseg, sb);
-    __syncthreads();
 
     if (final_pass) {
         // masked quotient + offset (interpolation.py:367, :428-430), in place in the v lines
