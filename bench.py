@@ -14,10 +14,11 @@ stream); cpu_baseline = the CPU oracle (FP64 C restatement of the reference path
 one core) on a bounded sample of the same fields.  'workloads' adds the other
 workloads, each with its throughput, pipeline roofline fraction and error against the
 FP64 path: cfg3 (256 uneven 2D fields), cfg4 (3D 512x512x128) and cfg4_batch (32 3D
-fields of 256x256x64), cfg5 (16 fields of 2400x1200, N=100000, forward + adjoint).
+fields of 256x256x64), cfg5 (16 fields of 2400x1200, N=100000, forward + adjoint), real
+(the RMSE-versus-iterations curve on the 3490 real stations of tests/golden/qff).
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--fields 64] [--cpu-seconds 10]
-                    [--legs cfg3,cfg4,cfg5|none] [--leg-steps 10]
+                    [--legs cfg3,cfg4,cfg5,real|none] [--leg-steps 10]
 """
 import argparse
 import json
@@ -194,6 +195,29 @@ def leg_train(cfgs, steps, warmup):
     return res
 
 
+def leg_real(steps):
+    """The paper's RMSE-versus-iterations curve (figure 8) on the 3490 real QFF stations at
+    2400x1200: num_iter 1..50 for both convolution methods, each method one FP32C batch of 50
+    fields, against naive Barnes (GPU, FP64); the FP64 path's curve beside it."""
+    from fastbarnes import hip, realdata
+    log('real: figure-8 curve, 3490 stations, 2400x1200')
+    pts, val = realdata.read_csv_array(os.path.join(ROOT, 'tests', 'golden', 'qff', 'PressQFF_202007271200_3490.csv'))
+    ns = list(range(1, 51))
+    t0 = time.perf_counter()
+    c32, naive = realdata.rmse_curve(pts, val, 32.0, num_iters=ns, precision='fp32c')
+    hip.synchronize()
+    t_curve = time.perf_counter() - t0
+    c64, _ = realdata.rmse_curve(pts, val, 32.0, num_iters=ns, precision='fp64', naive=naive)
+    pick = [1, 2, 3, 4, 5, 10, 20, 50]
+    diff = max(abs(a - b) for m in c32 for a, b in zip(c32[m], c64[m]))
+    return dict(workload='3490 real QFF stations, paper grid 2400x1200 (x0=(-26+1/32, 34.5)), sigma=1; '
+                         'RMSE over [-7,5]x[36,56] against naive Barnes',
+                seconds_whole_curve_fp32c_incl_naive_and_host_io=round(t_curve, 3),
+                rmse_conv={n: round(c32['convolution'][n - 1], 5) for n in pick},
+                rmse_opt_conv={n: round(c32['optimized_convolution'][n - 1], 5) for n in pick},
+                max_abs_rmse_diff_fp32c_vs_fp64=diff)
+
+
 def run_legs(names, args):
     from fastbarnes import workloads
     legs = {}
@@ -211,6 +235,8 @@ def run_legs(names, args):
         legs['cfg4_batch'] = leg_batch('cfg4_batch', cfgs, args.leg_steps, 2, 2,
                                        '32 fields 3D 256x256x64, N~logU[1e4,2e5], step 0.125, sigma=1.5, n=4; '
                                        'fp32c, hipGraph replay')
+    if 'real' in names:
+        legs['real'] = leg_real(args.leg_steps)
     if 'cfg5' in names:
         cfgs = [workloads.config5(seed=500 + i) for i in range(16)]
         legs['cfg5'] = leg_train(cfgs, args.leg_steps, 2)
@@ -226,8 +252,9 @@ def main():
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-extras', action='store_true', help='skip accuracy and FP64-path legs')
-    ap.add_argument('--legs', default='cfg3,cfg4,cfg5',
-                    help="comma list of extra workloads reported under 'workloads' (cfg3, cfg4, cfg5) or 'none'")
+    ap.add_argument('--legs', default='cfg3,cfg4,cfg5,real',
+                    help="comma list of extra workloads reported under 'workloads' (cfg3, cfg4, cfg5, real) "
+                         "or 'none'")
     ap.add_argument('--leg-steps', type=int, default=10)
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='after the run, one device-to-device copy of MIB MiB (a plain copy kernel of known '

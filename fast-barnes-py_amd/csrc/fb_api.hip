@@ -190,6 +190,22 @@ hipStream_t lib_stream(int device)
     return s;
 }
 
+// Grouped FP32C normalisation of a pass with per-iteration gain g = K + 2*alpha: groups of
+// m iterations with g^m <= 2^60, so values stored in float32 between iterations stay far
+// below its range (2^128) for any num_iter; a single group (scaling only after the last
+// iteration) whenever g^num_iter <= 2^60.
+void scale_groups(double g, int n, double total, PassField &p)
+{
+    int m = n;
+    if (g > 1.0) m = std::max(1, std::min(n, (int)std::floor(60.0 * std::log(2.0) / std::log(g))));
+    const int ngroups = (n + m - 1) / m;
+    const int last = n - m * (ngroups - 1);
+    p.pscale = total;
+    p.sgroup = m;
+    p.sgscale = ngroups > 1 ? std::pow(1.0 / g, (double)m) : 1.0;
+    p.slast = ngroups > 1 ? std::pow(1.0 / g, (double)last) : total;
+}
+
 // lanes per line / lines per workgroup of a fast line pass
 int pass_geometry(int64_t L, int T, int &lpl, int &lines, int &rpad, size_t &lds)
 {
@@ -392,7 +408,10 @@ int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<Pa
         p.lines = lines;
         p.rpad = rpad;
         p.alpha = F.alpha[axis];
-        p.pscale = F.pscale[axis];
+        if (F.pscale[axis] != 1.0)
+            scale_groups((double)(2 * F.T[axis] + 1) + 2 * F.alpha[axis], F.num_iter, F.pscale[axis], p);
+        else
+            scale_groups(1.0, F.num_iter, 1.0, p);
         std::vector<PassWork> items;
         if (axis == 0) {
             p.line_stride = W;
@@ -947,7 +966,7 @@ int fb_accumulate_lines(int tail, int precision, int64_t nlines, int64_t len, in
         pf.lines = lines;
         pf.rpad = rpad;
         pf.alpha = F.alpha[0];
-        pf.pscale = 1.0;
+        scale_groups(1.0, num_iter, 1.0, pf);
         std::vector<PassWork> work;
         for (int64_t l0 = 0; l0 < nlines; l0 += lines)
             work.push_back({0, (int32_t)std::min<int64_t>(lines, nlines - l0), l0 * len});

@@ -55,7 +55,14 @@ struct PassField {
     int32_t rpad;          // zeros after each line in LDS (>= T+1)
     int32_t lines;         // line capacity of a workgroup (lines * lpl <= 256)
     double alpha;
-    double pscale;         // applied to the pass result
+    double pscale;         // (1/(K+2*alpha))^num_iter: the pass's total normalisation
+    // The normalisation is applied in groups so that float32 storage between iterations
+    // cannot overflow: after every sgroup-th iteration multiply by sgscale, after the last
+    // one by slast (the product of all applied factors is pscale).
+    int32_t sgroup;
+    int32_t pad1;
+    double sgscale;
+    double slast;
 };
 
 // Contribution record of the sorted injection list.

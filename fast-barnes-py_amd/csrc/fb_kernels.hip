@@ -611,8 +611,9 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     float sb[kSegArr];
     for (int it = 0; it < niter; ++it) {
         const bool last = it + 1 == niter;
+        const bool grp = (it + 1) % P.sgroup == 0;
         if (act) {
-            if (last) slide1<true>(ln, s0, T, alpha, P.pscale, sb);
+            if (last || grp) slide1<true>(ln, s0, T, alpha, last ? P.slast : P.sgscale, sb);
             else slide1<false>(ln, s0, T, alpha, 1.0, sb);
         }
         __syncthreads();
