@@ -77,6 +77,7 @@ def cpu_baseline(cfgs, seconds):
 
 # algorithmic bytes per grid point / per sample point of the other workloads' pipelines
 PIPE_BYTES = {2: (28.0, 24.0), 3: (52.0, 32.0)}
+AUTOTUNE = True   # --no-autotune: keep the cost model's segment lengths
 # training (2D): forward also keeps the weight field (+8 B/pt); backward reads g, f, W (16),
 # round-trips two float32 fields (8 + 8), reads pts/val and writes the gradients (24 + 24)
 TRAIN_BYTES = (36.0 + 32.0, 24.0 + 48.0)
@@ -121,6 +122,10 @@ def leg_batch(name, cfgs, steps, warmup, check_fields, note):
     d_pts, d_val = hip.DeviceArray.from_numpy(pts), hip.DeviceArray.from_numpy(val)
     d_out = hip.DeviceArray((batch.total_out,), np.float32)
     s = hip.Stream()
+    tuned = None
+    if AUTOTUNE:
+        batch.autotune(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr)
+        tuned = {'axis%d' % a: batch.pass_groups(a) for a in range(batch.dim)}
     ms = _time_steps(lambda: batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr, graph=True), s, steps, warmup)
     batch.set_timing(True)
     acc = {}
@@ -135,7 +140,8 @@ def leg_batch(name, cfgs, steps, warmup, check_fields, note):
     res = dict(workload=note, fields=len(cfgs), grid_points=npts, sample_points=nsamp,
                value=round(npts / (ms * 1e-3), 1), unit='grid-pts/s', ms_per_step=round(ms, 4),
                pipeline_alg_bytes=pipe, pipeline_frac=round(pipe / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-               stage_ms={k: round(float(np.mean(v)), 5) for k, v in acc.items()})
+               stage_ms={k: round(float(np.mean(v)), 5) for k, v in acc.items()},
+               autotuned_segments=tuned)
     out = d_out.numpy()
     b64 = BarnesBatch(descs[:check_fields], precision='fp64')
     d_o64 = hip.DeviceArray((b64.total_out,), np.float32)
@@ -256,10 +262,14 @@ def main():
                     help="comma list of extra workloads reported under 'workloads' (cfg3, cfg4, cfg5, real) "
                          "or 'none'")
     ap.add_argument('--leg-steps', type=int, default=10)
+    ap.add_argument('--no-autotune', action='store_true',
+                    help="keep the cost model's segment lengths instead of timing the variants")
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='after the run, one device-to-device copy of MIB MiB (a plain copy kernel of known '
                          'size that scales HBM byte counters)')
     args = ap.parse_args()
+    global AUTOTUNE
+    AUTOTUNE = not args.no_autotune
     if args.gpus != 1:
         print('bench.py measures one GPU only: this build has no multi-GPU path (fields shard across GPUs '
               'with no communication; see DESIGN.md). Run with --gpus 1.', file=sys.stderr)
@@ -281,6 +291,11 @@ def main():
     stream = hip.Stream()
     npts = batch.total_out
     nsamples = batch.total_points
+    tuned = None
+    if AUTOTUNE:
+        log('autotune')
+        batch.autotune(d_pts.ptr, d_val.ptr, d_out.ptr, stream.ptr)
+        tuned = {'axis%d' % a: batch.pass_groups(a) for a in range(batch.dim)}
 
     log('warmup %d' % args.warmup)
     for _ in range(args.warmup):
@@ -391,7 +406,7 @@ def main():
         'config': {'workload': 'batch of %d fields, 2400x1200 grid, N=3490 stations each, sigma=1, num_iter=4, '
                                'max_dist=3.5, optimized_convolution, fp32c, hipGraph replay' % args.fields,
                    'fields': args.fields, 'grid': [2400, 1200], 'stations_per_field': 3490, 'num_iter': 4,
-                   'grid_points_per_step': npts},
+                   'grid_points_per_step': npts, 'autotuned_segments': tuned},
         'roofline': roof, 'pipeline_roofline': pipeline, 'cpu_baseline': cpu,
         'wall_s_timed': round(wall, 4),
     }

@@ -25,7 +25,6 @@ namespace {
 constexpr int kBlock = 256;             // lanes per field of a fast-pass workgroup (the
                                          // workgroup holds kBlock v workers + kBlock w workers)
 constexpr int kFastThreads = 2 * kBlock;
-constexpr int kSegLen = 41;              // elements per lane in the fast passes (kSegArr of the kernel)
 constexpr size_t kLdsBudget = 80 * 1024; // two workgroups per CU
 constexpr size_t kLdsMax = 160 * 1024;
 
@@ -206,16 +205,16 @@ void scale_groups(double g, int n, double total, PassField &p)
     p.slast = ngroups > 1 ? std::pow(1.0 / g, (double)last) : total;
 }
 
-// lanes per line / lines per workgroup of a fast line pass
-int pass_geometry(int64_t L, int T, int &lpl, int &lines, int &rpad, size_t &lds)
+// lanes per line / lines per workgroup of a fast line pass with `seg` elements per lane
+int pass_geometry(int64_t L, int T, int seg, int &lpl, int &lines, int &rpad, size_t &lds)
 {
-    const int64_t need = (L + kSegLen - 1) / kSegLen;
+    const int64_t need = (L + seg - 1) / seg;
     lpl = 1;
     while (lpl < need) lpl <<= 1;
     if (lpl > kBlock)
         return fail(FB_ERR_UNSUPPORTED, "line length " + std::to_string(L) + " exceeds the FP32C limit of " +
-                                            std::to_string(kBlock * kSegLen));
-    // branch-free slides read up to kSegLen-1 floats past the last w line: reserve 64 floats
+                                            std::to_string(kBlock * seg));
+    // branch-free slides read up to seg-1 floats past the last w line: reserve 64 floats
     const size_t reserve = 64 * sizeof(float);
     const int lane_lines = kBlock / lpl;
     auto fit = [&](int rp) {
@@ -226,11 +225,11 @@ int pass_geometry(int64_t L, int T, int &lpl, int &lines, int &rpad, size_t &lds
         return (int)std::max<int64_t>(n, 1);
     };
     // Zeros after each line: the reference's T+1.  A line of several segments may instead
-    // widen them to the overflow of its last segment (kSegLen - L mod kSegLen), so that the
+    // widen them to the overflow of its last segment (seg - L mod seg), so that the
     // lane whose segment crosses the line end stores all of it and the pass re-zeroes the
     // padding, instead of predicating every store.  The wider padding is taken when it does
     // not cost lines per workgroup.
-    const int over = (int)((kSegLen - L % kSegLen) % kSegLen);
+    const int over = (int)((seg - L % seg) % seg);
     rpad = T + 1;
     lines = fit(rpad);
     if (need > 1 && over > T + 1) {
@@ -246,6 +245,41 @@ int pass_geometry(int64_t L, int T, int &lpl, int &lines, int &rpad, size_t &lds
         return fail(FB_ERR_UNSUPPORTED, "line of length " + std::to_string(L) + " with half kernel " +
                                             std::to_string(T) + " does not fit the local data share");
     return FB_OK;
+}
+
+// Segment lengths of the fast-pass instantiations (FB_FAST_SEGS of fb_kernels.hip).
+constexpr int kSegs[] = {25, 29, 33, 37, 41, 45, 49};
+
+// Cost model of a fast pass per line: every lane of a workgroup issues the same
+// instructions, about 7 per element of its segment (slide), 2 per window term of the
+// re-anchoring (2T+1 terms) and a fixed ~30 per iteration, plus 2 per element when the
+// last segment's stores are predicated; a workgroup holds `lines` lines.
+double pass_cost(int64_t L, int T, int seg, bool *ok = nullptr)
+{
+    int lpl, lines, rpad;
+    size_t lds;
+    const std::string keep = last_error();
+    const int rc = pass_geometry(L, T, seg, lpl, lines, rpad, lds);
+    set_error(keep);
+    if (ok) *ok = rc == FB_OK;
+    if (rc) return 1e300;
+    const int over = (int)((seg - L % seg) % seg);
+    const bool full = L > seg && rpad >= over;
+    return (7.0 * seg + 2.0 * (2.0 * T + 1.0) + 30.0 + (full ? 0.0 : 2.0 * seg)) / lines;
+}
+
+int model_seg(int64_t L, int T)
+{
+    int best = 41;
+    double bc = 1e300;
+    for (int seg : kSegs) {
+        const double c = pass_cost(L, T, seg);
+        if (c < bc) {
+            bc = c;
+            best = seg;
+        }
+    }
+    return best;
 }
 
 // order work items so that blockIdx b runs tile t with consecutive tiles on one XCD
@@ -277,6 +311,18 @@ using namespace fb;
 
 // ---- plan --------------------------------------------------------------------
 
+namespace fb {
+namespace {
+// one launch of a fast pass: work items [first, first + count) of the axis's list
+struct PassGroup {
+    int seg;
+    int first;
+    int count;
+    size_t lds;
+};
+}  // namespace
+}  // namespace fb
+
 struct fb_plan {
     int device = 0;
     int precision = FB_PREC_FP64;
@@ -297,9 +343,10 @@ struct fb_plan {
     double *d_vg = nullptr, *d_wg = nullptr, *d_vs = nullptr, *d_ws = nullptr;
     float *d_v = nullptr, *d_w = nullptr;
     PassWork *d_work[3] = {nullptr, nullptr, nullptr};
-    int nwork[3] = {0, 0, 0};
     PassField *d_pass[3] = {nullptr, nullptr, nullptr};
-    size_t lds[3] = {0, 0, 0};
+    std::vector<PassGroup> groups[3];          // launches of each axis pass, one per segment length
+    std::map<std::pair<int, int64_t>, int> seg_choice;   // (axis, line length) -> segment (autotuned)
+    const uint32_t *sorted_vals = nullptr;      // sorted point indices of the last run
 
     int timing = 0;
     std::vector<hipEvent_t> events;
@@ -379,6 +426,11 @@ int make_field(const fb_field &f, FieldDev &d, int fast)
         if ((int64_t)K[m] >= f.size[m] || d.T[m] == INT32_MAX)
             return fail(FB_ERR_INVALID_ARGUMENT,
                         "resulting rectangular kernel size should be smaller w.r.t. specified grid");
+        if (fast && !d.tail && std::pow((double)K[m], (double)f.num_iter) >= 9.2233720368547758e18)
+            return fail(FB_ERR_UNSUPPORTED,
+                        "FP32C does not support method convolution with kernel_size**num_iter >= 2**63: the "
+                        "reference's integer threshold wraps around there and leaves nodes with weights below "
+                        "the float32 range unmasked; use precision fp64");
         d.alpha[m] = d.tail ? tail_value(f.sigma[m], f.step[m], f.num_iter) : 0.0;
         d.pscale[m] = fast ? std::pow(1.0 / ((double)K[m] + 2 * d.alpha[m]), (double)f.num_iter) : 1.0;
         prod_scale *= d.pscale[m];
@@ -388,31 +440,39 @@ int make_field(const fb_field &f, FieldDev &d, int fast)
     return FB_OK;
 }
 
-int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<PassField> &pf, size_t &lds)
+// Work list and per-field geometry of the fast pass along `axis`: each field runs with the
+// segment length chosen for its line length (autotuned choice, else the cost model); the
+// work items are grouped by segment length, one launch per group.
+int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<PassField> &pf,
+               std::vector<PassGroup> &groups)
 {
-    lds = 0;
     pf.assign(P->nfields, PassField{});
+    std::map<int, std::vector<PassWork>> by_seg;
+    std::map<int, size_t> lds_of;
     for (int f = 0; f < P->nfields; ++f) {
         const FieldDev &F = P->fh[f];
         const int64_t W = F.size[0], H = F.size[1], D = F.size[2];
         const int64_t L = F.size[axis];
+        auto it = P->seg_choice.find({axis, L});
+        const int seg = it != P->seg_choice.end() ? it->second : model_seg(L, F.T[axis]);
         int lpl = 1, lines = 1, rpad = 0;
         size_t l = 0;
-        int rc = pass_geometry(L, F.T[axis], lpl, lines, rpad, l);
+        int rc = pass_geometry(L, F.T[axis], seg, lpl, lines, rpad, l);
         if (rc) return rc;
-        lds = std::max(lds, l);
+        lds_of[seg] = std::max(lds_of[seg], l);
         PassField &p = pf[f];
         p.len = L;
         p.T = F.T[axis];
         p.lpl = lpl;
         p.lines = lines;
         p.rpad = rpad;
+        p.seg = seg;
         p.alpha = F.alpha[axis];
         if (F.pscale[axis] != 1.0)
             scale_groups((double)(2 * F.T[axis] + 1) + 2 * F.alpha[axis], F.num_iter, F.pscale[axis], p);
         else
             scale_groups(1.0, F.num_iter, 1.0, p);
-        std::vector<PassWork> items;
+        std::vector<PassWork> &items = by_seg[seg];
         if (axis == 0) {
             p.line_stride = W;
             p.elem_stride = 1;
@@ -425,17 +485,48 @@ int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<Pa
             for (int64_t z = 0; z < D; ++z)
                 for (int64_t x0 = 0; x0 < W; x0 += lines)
                     items.push_back({f, (int32_t)std::min<int64_t>(lines, W - x0), z * W * H + x0});
-            items = xcd_order(items);
         } else {
             p.line_stride = 1;
             p.elem_stride = W * H;
             for (int64_t c0 = 0; c0 < W * H; c0 += lines)
                 items.push_back({f, (int32_t)std::min<int64_t>(lines, W * H - c0), c0});
-            items = xcd_order(items);
         }
+    }
+    work.clear();
+    groups.clear();
+    for (auto &kv : by_seg) {
+        std::vector<PassWork> items = axis == 0 ? kv.second : xcd_order(kv.second);
+        groups.push_back({kv.first, (int)work.size(), (int)items.size(), lds_of[kv.first]});
         work.insert(work.end(), items.begin(), items.end());
     }
     if (work.size() > (size_t)INT32_MAX) return fail(FB_ERR_UNSUPPORTED, "too many work items");
+    return FB_OK;
+}
+
+// (Re)builds and uploads the fast passes of all axes of an FP32C plan.
+int build_passes(fb_plan *P)
+{
+    for (int a = 0; a < P->dim; ++a) {
+        std::vector<PassWork> work;
+        std::vector<PassField> pf;
+        int rc = build_pass(P, a, work, pf, P->groups[a]);
+        if (rc) return rc;
+        if ((rc = P->upload(&P->d_work[a], work))) return rc;
+        if ((rc = P->upload(&P->d_pass[a], pf))) return rc;
+    }
+    return FB_OK;
+}
+
+// One fast pass along `axis` over the whole batch: a launch per segment-length group.
+int run_pass(fb_plan *P, int axis, bool inject, const uint32_t *vals, const double *pts, const double *val,
+             float *out, float *wsave, int final_pass, hipStream_t s)
+{
+    for (const PassGroup &g : P->groups[axis]) {
+        int rc = launch_fast_pass(inject, g.seg, P->d_fields, P->d_offsets, P->d_work[axis] + g.first, g.count,
+                                  P->d_pass[axis], kFastThreads, g.lds, nullptr, vals, P->d_line_start,
+                                  P->d_line_count, pts, val, P->d_v, P->d_w, out, wsave, final_pass, s);
+        if (rc) return rc;
+    }
     return FB_OK;
 }
 
@@ -486,7 +577,7 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
     xline_prefix.push_back(P->nxlines);
     cell_prefix.push_back(P->ncells);
     for (int a = 0; a < 3; ++a) lslot[a].push_back(P->lslots[a]);
-    P->nentries = P->npoints * P->corners;
+    P->nentries = P->npoints;   // one (base cell, point) pair per point
     if (P->ncells >= (int64_t)kInvalidKey || P->npoints >= (int64_t)UINT32_MAX || P->nentries >= (int64_t)UINT32_MAX) {
         delete P;
         return fail(FB_ERR_UNSUPPORTED, "a plan holds at most 2^32-2 grid cells and 2^32-1 contributions");
@@ -520,14 +611,7 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
     } else {
         TRY(P->alloc(&P->d_v, P->ncells));
         TRY(P->alloc(&P->d_w, P->ncells));
-        for (int a = 0; a < P->dim; ++a) {
-            std::vector<PassWork> work;
-            std::vector<PassField> pf;
-            TRY(build_pass(P, a, work, pf, P->lds[a]));
-            P->nwork[a] = (int)work.size();
-            TRY(P->upload(&P->d_work[a], work));
-            TRY(P->upload(&P->d_pass[a], pf));
-        }
+        TRY(build_passes(P));
     }
 #undef TRY
     *out = P;
@@ -553,19 +637,20 @@ int plan_run(fb_plan *P, const double *pts, const double *val, float *out, float
     stage_mark(P, s, "begin");
     if ((rc = launch_zero(P->d_line_count, P->nxlines * 4, s))) return rc;
     if ((rc = launch_offsets(P->d_fields, P->nfields, val, P->d_offsets, s))) return rc;
-    if ((rc = launch_contrib(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints, P->corners, pts, P->d_keys,
-                             P->d_vals, P->d_line_count, s)))
+    if ((rc = launch_point_keys(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints, pts, P->d_keys, P->d_vals,
+                                P->d_line_count, s)))
         return rc;
     uint32_t *ks = nullptr, *vs = nullptr;
     if ((rc = radix_sort_pairs(P->d_keys, P->d_vals, P->d_keys2, P->d_vals2, P->nentries, P->key_bits, P->d_hist,
                                P->d_scan_tmp, s, &ks, &vs)))
         return rc;
     if ((rc = scan_exclusive(P->d_line_count, P->d_line_start, P->nxlines, P->d_scan_tmp, s))) return rc;
+    P->sorted_vals = vs;
     stage_mark(P, s, "sort");
     if (P->precision == FB_PREC_FP64) {
         if ((rc = launch_zero(P->d_vg, P->ncells * 8, s))) return rc;
         if ((rc = launch_zero(P->d_wg, P->ncells * 8, s))) return rc;
-        if ((rc = launch_exact_inject(P->d_fields, P->d_xline_prefix, P->nfields, P->nxlines, ks, vs, P->d_line_start,
+        if ((rc = launch_exact_inject(P->d_fields, P->d_xline_prefix, P->nfields, P->nxlines, vs, P->d_line_start,
                                       P->d_line_count, pts, val, P->d_offsets, P->d_vg, P->d_wg, s)))
             return rc;
         stage_mark(P, s, "inject");
@@ -584,10 +669,7 @@ int plan_run(fb_plan *P, const double *pts, const double *val, float *out, float
         static const char *pn[3] = {"pass_x", "pass_y", "pass_z"};
         for (int a = 0; a < P->dim; ++a) {
             const int fin = a == P->dim - 1;
-            if ((rc = launch_fast_pass(a == 0, P->d_fields, P->d_offsets, P->d_work[a], P->nwork[a], P->d_pass[a],
-                                       kFastThreads, P->lds[a], ks, vs, P->d_line_start, P->d_line_count, pts, val,
-                                       P->d_v, P->d_w, out, fin ? wsave : nullptr, fin, s)))
-                return rc;
+            if ((rc = run_pass(P, a, a == 0, vs, pts, val, out, fin ? wsave : nullptr, fin, s))) return rc;
             stage_mark(P, s, pn[a]);
         }
     }
@@ -750,12 +832,116 @@ int fb_plan_backward(fb_plan *P, const double *d_pts, const double *d_val, const
         return rc;
     // the same scaled line passes, reading both fields from HBM on every axis
     for (int a = 0; a < P->dim; ++a)
-        if ((rc = launch_fast_pass(false, P->d_fields, P->d_offsets, P->d_work[a], P->nwork[a], P->d_pass[a],
-                                   kFastThreads, P->lds[a], nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                   P->d_v, P->d_w, nullptr, nullptr, 0, s)))
-            return rc;
+        if ((rc = run_pass(P, a, false, nullptr, nullptr, nullptr, nullptr, nullptr, 0, s))) return rc;
     return launch_adj_gather(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints, d_pts, d_val, P->d_offsets,
                              P->d_v, P->d_w, d_gval, d_gpts, s);
+}
+
+// Device time (ms, mean of 3 after a warm-up) of the work items of the fields selected by
+// `sel` in the pass `work`/`pf`/`groups` along axis a, on the plan's own buffers.
+static int time_pass_subset(fb_plan *P, int a, const std::vector<PassWork> &work, const std::vector<PassField> &pf,
+                            const std::vector<PassGroup> &groups, const std::vector<char> &sel, const double *pts,
+                            const double *val, float *out, hipStream_t s, float *ms)
+{
+    std::vector<PassWork> fw;
+    std::vector<PassGroup> fg;
+    for (const PassGroup &g : groups) {
+        const int first = (int)fw.size();
+        for (int i = g.first; i < g.first + g.count; ++i)
+            if (sel[work[i].field]) fw.push_back(work[i]);
+        if ((int)fw.size() > first) fg.push_back({g.seg, first, (int)fw.size() - first, g.lds});
+    }
+    *ms = 0.f;
+    if (fw.empty()) return FB_OK;
+    Scratch sc;
+    sc.device = P->device;
+    PassWork *dw;
+    PassField *dp;
+    int rc;
+    if ((rc = sc.get(&dw, fw.size())) || (rc = sc.get(&dp, pf.size()))) return rc;
+    FB_HIP_CHECK(hipMemcpy(dw, fw.data(), fw.size() * sizeof(PassWork), hipMemcpyHostToDevice));
+    FB_HIP_CHECK(hipMemcpy(dp, pf.data(), pf.size() * sizeof(PassField), hipMemcpyHostToDevice));
+    hipEvent_t e0, e1;
+    FB_HIP_CHECK(hipEventCreate(&e0));
+    FB_HIP_CHECK(hipEventCreate(&e1));
+    const int fin = a == P->dim - 1;
+    for (int rep = 0; rep < 4 && !rc; ++rep) {
+        if (rep == 1) (void)hipEventRecord(e0, s);
+        for (const PassGroup &g : fg) {
+            rc = launch_fast_pass(a == 0, g.seg, P->d_fields, P->d_offsets, dw + g.first, g.count, dp, kFastThreads,
+                                  g.lds, nullptr, P->sorted_vals, P->d_line_start, P->d_line_count, pts, val, P->d_v,
+                                  P->d_w, out, nullptr, fin, s);
+            if (rc) break;
+        }
+    }
+    (void)hipEventRecord(e1, s);
+    hipError_t e = hipEventSynchronize(e1);
+    if (!rc && e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(FB_ERR_HIP, std::string("autotune timing failed: ") + hipGetErrorString(e));
+    *ms /= 3.f;
+    return FB_OK;
+}
+
+int fb_plan_autotune(fb_plan *P, const double *d_pts, const double *d_val, float *d_out, void *stream)
+{
+    if (!P || !d_pts || !d_val || !d_out) return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (P->precision != FB_PREC_FP32C) return FB_OK;   // the FP64 path has nothing to tune
+    hipStream_t s = (hipStream_t)stream;
+    int rc = plan_run(P, d_pts, d_val, d_out, nullptr, s);   // sorted injection lists for the x pass
+    if (rc) return rc;
+    FB_HIP_CHECK(hipStreamSynchronize(s));
+    for (int a = 0; a < P->dim; ++a) {
+        std::map<int64_t, std::vector<char>> classes;   // line length -> fields with it
+        for (int f = 0; f < P->nfields; ++f) {
+            std::vector<char> &sel = classes[P->fh[f].size[a]];
+            sel.resize(P->nfields, 0);
+            sel[f] = 1;
+        }
+        for (auto &cl : classes) {
+            int best = -1;
+            float best_ms = 0.f;
+            for (int seg : kSegs) {
+                bool ok = true;
+                for (int f = 0; f < P->nfields && ok; ++f)
+                    if (cl.second[f]) pass_cost(cl.first, P->fh[f].T[a], seg, &ok);
+                if (!ok) continue;
+                P->seg_choice[{a, cl.first}] = seg;
+                std::vector<PassWork> work;
+                std::vector<PassField> pf;
+                std::vector<PassGroup> groups;
+                if ((rc = build_pass(P, a, work, pf, groups))) return rc;
+                float ms = 0.f;
+                if ((rc = time_pass_subset(P, a, work, pf, groups, cl.second, d_pts, d_val, d_out, s, &ms)))
+                    return rc;
+                if (best < 0 || ms < best_ms) {
+                    best = seg;
+                    best_ms = ms;
+                }
+            }
+            if (best < 0) P->seg_choice.erase({a, cl.first});
+            else P->seg_choice[{a, cl.first}] = best;
+        }
+    }
+    if ((rc = build_passes(P))) return rc;
+    if (P->gexec) { (void)hipGraphExecDestroy(P->gexec); P->gexec = nullptr; }
+    if (P->graph) { (void)hipGraphDestroy(P->graph); P->graph = nullptr; }
+    return FB_OK;
+}
+
+int fb_plan_pass_groups(fb_plan *P, int axis, int32_t *seg, int32_t *items, int max_groups, int *ngroups)
+{
+    if (!P || !ngroups) return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (axis < 0 || axis > 2) return fail(FB_ERR_INVALID_ARGUMENT, "axis must be 0, 1 or 2");
+    const std::vector<PassGroup> &g = P->groups[axis];
+    *ngroups = (int)g.size();
+    for (int i = 0; i < (int)g.size() && i < max_groups; ++i) {
+        if (seg) seg[i] = g[i].seg;
+        if (items) items[i] = g[i].count;
+    }
+    return FB_OK;
 }
 
 int fb_plan_set_timing(fb_plan *P, int enable)
@@ -903,6 +1089,112 @@ int fb_barnes(int method, int precision, int dim, int64_t n, const double *pts, 
     return rc;
 }
 
+int fb_barnes_s2(int method, int precision, int64_t n, const double *pts, const double *val, const double *sigma,
+                 const double *x0, const double *step, const int64_t *size, int num_iter, double max_dist_weight,
+                 int resample, void *out, unsigned flags, void *stream)
+{
+    if (n < 1 || !pts || !val || !sigma || !x0 || !step || !size || !out)
+        return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument or no sample points");
+    if (method != FB_METHOD_OPTIMIZED_CONVOLUTION && method != FB_METHOD_NAIVE)
+        return fail(FB_ERR_INVALID_ARGUMENT, "encountered invalid Barnes interpolation method");
+    for (int m = 0; m < 2; ++m)
+        if (size[m] < 1 || !(step[m] > 0.0) || !(sigma[m] > 0.0) || !std::isfinite(x0[m]))
+            return fail(FB_ERR_INVALID_ARGUMENT, "grid size, step and sigma must be positive, x0 finite");
+    const bool on_dev = flags & FB_FLAG_DEVICE_PTRS;
+    int device = 0;
+    FB_HIP_CHECK(hipGetDevice(&device));
+    hipStream_t s = stream ? (hipStream_t)stream : lib_stream(device);
+    Scratch sc;
+    sc.device = device;
+    int rc;
+    const double *d_pts = pts, *d_val = val;
+    if (on_dev) {
+        if ((rc = check_device_ptr(pts, "pts")) || (rc = check_device_ptr(val, "val")) ||
+            (rc = check_device_ptr(out, "out")))
+            return rc;
+    } else {
+        double *a, *b;
+        if ((rc = sc.get(&a, n * 2)) || (rc = sc.get(&b, n))) return rc;
+        FB_HIP_CHECK(hipMemcpyAsync(a, pts, n * 2 * sizeof(double), hipMemcpyHostToDevice, s));
+        FB_HIP_CHECK(hipMemcpyAsync(b, val, n * sizeof(double), hipMemcpyHostToDevice, s));
+        d_pts = a;
+        d_val = b;
+    }
+    // get_lambert_proj: create_proj(11.5, 34.5, 42.5, 65.5), lambert_conformal.py:50-97
+    double proj[5];
+    {
+        const double rad = 3.141592653589793 / 180.0, h = rad / 2.0;
+        const double lat1 = 42.5, lat2 = 65.5, clat = 34.5;
+        const double nn = std::log(std::cos(lat1 * rad) / std::cos(lat2 * rad)) /
+                          std::log(std::tan((90.0 + lat2) * h) / std::tan((90.0 + lat1) * h));
+        const double F = std::cos(lat1 * rad) * std::pow(std::tan((90.0 + lat1) * h), nn) / nn;
+        proj[0] = 11.5;
+        proj[1] = nn;
+        proj[2] = 1.0 / nn;
+        proj[3] = F;
+        proj[4] = F / std::pow(std::tan((90.0 + clat) * h), nn);
+    }
+    const int64_t cells = size[0] * size[1];
+    if (method == FB_METHOD_NAIVE) {
+        FieldDev F;
+        std::memset(&F, 0, sizeof(F));
+        F.dim = 2;
+        F.n = n;
+        FieldDev *d_f;
+        double *d_off, *d_out = (double *)out;
+        if ((rc = sc.get(&d_f, 1)) || (rc = sc.get(&d_off, 1))) return rc;
+        if (!on_dev && (rc = sc.get(&d_out, cells))) return rc;
+        FB_HIP_CHECK(hipMemcpy(d_f, &F, sizeof(F), hipMemcpyHostToDevice));
+        if ((rc = launch_offsets(d_f, 1, d_val, d_off, s))) return rc;
+        if ((rc = launch_naive_s2(d_pts, d_val, d_off, n, sigma[0], x0, step, size, d_out, s))) return rc;
+        if (!on_dev) FB_HIP_CHECK(hipMemcpyAsync(out, d_out, cells * sizeof(double), hipMemcpyDeviceToHost, s));
+        FB_HIP_CHECK(hipStreamSynchronize(s));
+        return FB_OK;
+    }
+    // the fixed Lambert grid, interpolationS2.py:186-188
+    const double lam_x0[2] = {-32.0, -2.0};
+    const int64_t lam_size[2] = {(int64_t)(64.0 / step[0]), (int64_t)(44.0 / step[1])};
+    const int64_t lam_cells = lam_size[0] * lam_size[1];
+    double *d_lpts;
+    if ((rc = sc.get(&d_lpts, n * 2))) return rc;
+    if ((rc = launch_lambert_points(d_pts, d_lpts, n, proj, s))) return rc;
+    fb_field f;
+    std::memset(&f, 0, sizeof(f));
+    f.dim = 2;
+    f.method = FB_METHOD_OPTIMIZED_CONVOLUTION;
+    f.num_iter = num_iter;
+    f.n = n;
+    f.size[2] = 1;
+    for (int m = 0; m < 2; ++m) {
+        f.size[m] = lam_size[m];
+        f.sigma[m] = sigma[m];
+        f.x0[m] = lam_x0[m];
+        f.step[m] = step[m];
+    }
+    f.max_dist_weight = max_dist_weight;
+    fb_plan *P = nullptr;
+    if ((rc = plan_create(&P, &f, 1, precision))) return rc;
+    const int64_t out_cells = resample ? cells : lam_cells;
+    float *d_out = (float *)out, *d_lam = nullptr;
+    if (!on_dev) rc = sc.get(&d_out, out_cells);
+    if (!rc) {
+        if (resample) rc = sc.get(&d_lam, lam_cells);
+        else d_lam = d_out;
+    }
+    if (!rc) rc = plan_run(P, d_lpts, d_val, d_lam, nullptr, s);
+    if (!rc && resample) rc = launch_resample(d_lam, lam_size, lam_x0, x0, step, size, proj, d_out, s);
+    if (!rc && !on_dev) {
+        hipError_t e = hipMemcpyAsync(out, d_out, out_cells * sizeof(float), hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) rc = fail(FB_ERR_HIP, std::string("copy of the result failed: ") + hipGetErrorString(e));
+    }
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = fail(FB_ERR_HIP, std::string("execution failed: ") + hipGetErrorString(e));
+    }
+    delete P;
+    return rc;
+}
+
 int fb_accumulate_lines(int tail, int precision, int64_t nlines, int64_t len, int rect_len, int num_iter, double alpha,
                         const double *in, double *out, unsigned flags, void *stream)
 {
@@ -956,8 +1248,10 @@ int fb_accumulate_lines(int tail, int precision, int64_t nlines, int64_t len, in
         if ((rc = sc.get(&d_v, cells)) || (rc = sc.get(&d_w, cells)) || (rc = sc.get(&d_off, 1))) return rc;
         int lpl, lines, rpad;
         size_t lds;
-        if ((rc = pass_geometry(len, F.T[0], lpl, lines, rpad, lds))) return rc;
+        const int seg = model_seg(len, F.T[0]);
+        if ((rc = pass_geometry(len, F.T[0], seg, lpl, lines, rpad, lds))) return rc;
         PassField pf{};
+        pf.seg = seg;
         pf.len = len;
         pf.line_stride = len;
         pf.elem_stride = 1;
@@ -976,7 +1270,7 @@ int fb_accumulate_lines(int tail, int precision, int64_t nlines, int64_t len, in
         if ((rc = launch_zero(d_off, 8, s))) return rc;
         if ((rc = launch_f64_to_f32(d_io, d_v, cells, s))) return rc;
         if ((rc = launch_zero(d_w, cells * 4, s))) return rc;
-        if ((rc = launch_fast_pass(false, d_f, d_off, d_work, (int)work.size(), d_pf, kFastThreads, lds, nullptr, nullptr,
+        if ((rc = launch_fast_pass(false, seg, d_f, d_off, d_work, (int)work.size(), d_pf, kFastThreads, lds, nullptr, nullptr,
                                    nullptr, nullptr, nullptr, nullptr, d_v, d_w, nullptr, nullptr, 0, s)))
             return rc;
         if ((rc = launch_f32_to_f64(d_v, d_io, cells, 1.0, s))) return rc;
@@ -1060,16 +1354,18 @@ int fb_convolve_grid(int tail, int precision, int dim, const int64_t *size, cons
         for (int a = 0; a < dim; ++a) {
             std::vector<PassWork> work;
             std::vector<PassField> pf;
-            size_t lds = 0;
-            if ((rc = build_pass(&tmp, a, work, pf, lds))) return rc;
+            std::vector<PassGroup> groups;
+            if ((rc = build_pass(&tmp, a, work, pf, groups))) return rc;
             PassWork *d_work;
             PassField *d_pf;
             if ((rc = sc.get(&d_work, work.size())) || (rc = sc.get(&d_pf, 1))) return rc;
             FB_HIP_CHECK(hipMemcpy(d_work, work.data(), work.size() * sizeof(PassWork), hipMemcpyHostToDevice));
             FB_HIP_CHECK(hipMemcpy(d_pf, pf.data(), sizeof(PassField), hipMemcpyHostToDevice));
-            if ((rc = launch_fast_pass(false, d_f, d_off, d_work, (int)work.size(), d_pf, kFastThreads, lds, nullptr,
-                                       nullptr, nullptr, nullptr, nullptr, nullptr, d_v, d_w, nullptr, nullptr, 0, s)))
-                return rc;
+            for (const PassGroup &g : groups)
+                if ((rc = launch_fast_pass(false, g.seg, d_f, d_off, d_work + g.first, g.count, d_pf, kFastThreads,
+                                           g.lds, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, d_v, d_w,
+                                           nullptr, nullptr, 0, s)))
+                    return rc;
             FB_HIP_CHECK(hipStreamSynchronize(s));   // host vectors above go out of scope
         }
         if ((rc = launch_f32_to_f64(d_v, d_vg, cells, 1.0, s))) return rc;

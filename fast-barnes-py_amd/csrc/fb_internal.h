@@ -60,7 +60,7 @@ struct PassField {
     // cannot overflow: after every sgroup-th iteration multiply by sgscale, after the last
     // one by slast (the product of all applied factors is pscale).
     int32_t sgroup;
-    int32_t pad1;
+    int32_t seg;           // elements per lane (the kernel instantiation the pass runs)
     double sgscale;
     double slast;
 };

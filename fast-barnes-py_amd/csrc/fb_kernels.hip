@@ -2,13 +2,14 @@
 //
 // Pipeline of one batch (every stage covers all fields of the batch):
 //   k_offsets       value offset (min+max)/2 per field           interpolation.py:204-212
-//   k_contrib       per point: bilinear corner cells -> (cell key, point) pairs,
-//                   x-line counts                                 interpolation.py:218-322
-//   (fb_sort.hip)   stable radix sort of the pairs by cell, scan of the line counts
+//   k_point_keys    per point: base cell (lower corner) -> (cell key, point) pair,
+//                   per-row point counts                          interpolation.py:218-322
+//   (fb_sort.hip)   stable radix sort of the pairs by base cell, scan of the row counts
 //   FP64 path:      k_exact_inject, k_exact_lines (per axis), k_exact_finalize
 //                   -- the reference's operation order, one thread per line
 //   FP32C path:     k_fast_pass<inject> along x, k_fast_pass along y (and z);
 //                   quotient, offset and NaN mask fused into the last pass
+#include <climits>
 #include <cmath>
 #include <mutex>
 #include <string>
@@ -106,51 +107,63 @@ __device__ __forceinline__ double corner_weight(int dim, const PointPos &pp, int
     return w;
 }
 
-__global__ __launch_bounds__(kBlock) void k_contrib(const FieldDev *__restrict__ fields,
-                                                    const int64_t *__restrict__ pt_prefix, int nfields,
-                                                    int64_t npoints, int corners, const double *__restrict__ pts,
-                                                    uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                                                    uint32_t *__restrict__ line_count)
+// One (key, point) pair per point: the key is the point's base cell (the grid cell of its
+// lower corner), kInvalidKey for points the injection drops.  The corners of a point lie in
+// the base row (x-line) and the rows above it; counts per base row give, after the scan,
+// each row's range in the sorted pairs.
+__global__ __launch_bounds__(kBlock) void k_point_keys(const FieldDev *__restrict__ fields,
+                                                       const int64_t *__restrict__ pt_prefix, int nfields,
+                                                       int64_t npoints, const double *__restrict__ pts,
+                                                       uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                       uint32_t *__restrict__ row_count)
 {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= npoints) return;
     const int f = find_seg(pt_prefix, nfields, i);
     const FieldDev &F = fields[f];
-    const int dim = F.dim;
     PointPos pp;
-    const bool inside = point_pos(F, pts + i * dim, pp);
-    const int64_t slot = i * corners;
+    const bool inside = point_pos(F, pts + i * F.dim, pp);
+    vals[i] = (uint32_t)i;
     if (!inside) {
-        for (int cb = 0; cb < corners; ++cb) {
-            keys[slot + cb] = kInvalidKey;
-            vals[slot + cb] = (uint32_t)i;
-        }
+        keys[i] = kInvalidKey;
         return;
     }
-    const int64_t W = F.size[0], H = F.size[1];
-    for (int cb = 0; cb < corners; ++cb) {
-        const int64_t bx = cb & 1, by = (cb >> 1) & 1, bz = (cb >> 2) & 1;
-        const int64_t cell = ((pp.i[2] + bz) * H + (pp.i[1] + by)) * W + (pp.i[0] + bx);
-        keys[slot + cb] = (uint32_t)(F.cell_off + cell);
-        vals[slot + cb] = (uint32_t)i;
-    }
-    for (int q = 0; q < corners / 2; ++q) {
-        const int64_t by = q & 1, bz = q >> 1;
-        const int64_t line = (pp.i[2] + bz) * H + (pp.i[1] + by);
-        atomicAdd(&line_count[F.xline_off + line], 2u);
-    }
+    const int64_t row = pp.i[2] * F.size[1] + pp.i[1];
+    keys[i] = (uint32_t)(F.cell_off + row * F.size[0] + pp.i[0]);
+    atomicAdd(&row_count[F.xline_off + row], 1u);
+}
+
+// The sub-streams of x-line (y, z): for q < 2*nrows, the points of base row (y - dy, z - dz)
+// contribute their corner (bx, dy, dz) at x = base x + bx, with bx = q & 1, dy = (q >> 1) & 1,
+// dz = q >> 2.  Within a base row the sorted points ascend in base x (point order for equal
+// x), so every sub-stream ascends in target cell.
+__device__ __forceinline__ int source_rows(int dim) { return dim >= 3 ? 4 : (dim == 2 ? 2 : 1); }
+
+__device__ __forceinline__ bool sub_stream(const FieldDev &F, int64_t line, int q,
+                                           const uint32_t *__restrict__ row_start,
+                                           const uint32_t *__restrict__ row_count, uint32_t &rs, uint32_t &rc)
+{
+    const int dy = (q >> 1) & 1, dz = q >> 2;
+    const int64_t H = F.size[1];
+    const int64_t y = line % H, z = line / H;
+    rs = rc = 0;
+    if (y - dy < 0 || z - dz < 0) return false;
+    const int64_t row = line - dy - dz * H;
+    rs = row_start[F.xline_off + row];
+    rc = row_count[F.xline_off + row];
+    return true;
 }
 
 // ---------------------------------------------------------------------------
 // FP64 path, reference operation order
 
-// One thread per x-line: walks the line's contributions (sorted by cell, point
-// order inside a cell) and adds w*val_c and w to the FP64 grids in that order,
-// which is the per-cell order of the reference's point loop.
+// One thread per x-line: merges the line's sub-streams by (x, point index) and adds
+// w*val_c and w to the FP64 grids in that order, which is the per-cell order of the
+// reference's point loop (each cell gets its contributions in increasing point index).
 __global__ __launch_bounds__(kBlock) void k_exact_inject(
     const FieldDev *__restrict__ fields, const int64_t *__restrict__ xline_prefix, int nfields, int64_t nxlines,
-    const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals, const uint32_t *__restrict__ line_start,
-    const uint32_t *__restrict__ line_count, const double *__restrict__ pts, const double *__restrict__ val,
+    const uint32_t *__restrict__ vals, const uint32_t *__restrict__ row_start,
+    const uint32_t *__restrict__ row_count, const double *__restrict__ pts, const double *__restrict__ val,
     const double *__restrict__ offsets, double *__restrict__ vg, double *__restrict__ wg)
 {
     const int64_t gl = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -158,22 +171,59 @@ __global__ __launch_bounds__(kBlock) void k_exact_inject(
     const int f = find_seg(xline_prefix, nfields, gl);
     const FieldDev &F = fields[f];
     const int64_t line = gl - F.xline_off;
-    const int64_t W = F.size[0], H = F.size[1];
-    const int64_t y = line % H, z = line / H;
-    const uint32_t s = line_start[gl], c = line_count[gl];
+    const int64_t base = F.cell_off + line * F.size[0];
     const double off = offsets[f];
-    for (uint32_t e = s; e < s + c; ++e) {
-        const uint32_t key = keys[e];
-        const uint32_t p = vals[e];
-        const int64_t cell = (int64_t)key - F.cell_off;
-        const int64_t x = cell - line * W;
-        if ((uint64_t)x >= (uint64_t)W) continue;   // cannot happen for a consistent sort
+    const int nsub = 2 * source_rows(F.dim);
+    uint32_t pos[8], end[8], hp[8];
+    int hx[8];
+    double hw[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        pos[q] = end[q] = 0;
+        if (q < nsub) {
+            uint32_t rs, rc;
+            sub_stream(F, line, q, row_start, row_count, rs, rc);
+            pos[q] = rs;
+            end[q] = rs + rc;
+        }
+    }
+    // head of sub-stream q: target x, point, corner weight (INT_MAX when exhausted)
+    auto head = [&](int q, uint32_t e, int &x, uint32_t &p, double &w) {
+        if (e >= end[q]) {
+            x = INT_MAX;
+            p = 0;
+            w = 0.0;
+            return;
+        }
+        p = vals[e];
         PointPos pp;
         point_pos(F, pts + (int64_t)p * F.dim, pp);
-        const double w = corner_weight(F.dim, pp, x - pp.i[0], y - pp.i[1], z - pp.i[2]);
-        const double vc = val[p] - off;
-        vg[F.cell_off + cell] += w * vc;
-        wg[F.cell_off + cell] += w;
+        x = (int)pp.i[0] + (q & 1);
+        w = corner_weight(F.dim, pp, q & 1, (q >> 1) & 1, q >> 2);
+    };
+#pragma unroll
+    for (int q = 0; q < 8; ++q) head(q, pos[q], hx[q], hp[q], hw[q]);
+    while (true) {
+        int best = 0;
+#pragma unroll
+        for (int q = 1; q < 8; ++q)
+            if (hx[q] < hx[best] || (hx[q] == hx[best] && hp[q] < hp[best])) best = q;
+        int bx = INT_MAX;
+        uint32_t bp = 0;
+        double bw = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (q == best) {
+                bx = hx[q];
+                bp = hp[q];
+                bw = hw[q];
+                ++pos[q];
+                head(q, pos[q], hx[q], hp[q], hw[q]);
+            }
+        if (bx == INT_MAX) break;
+        const double vc = val[bp] - off;
+        vg[base + bx] += bw * vc;
+        wg[base + bx] += bw;
     }
 }
 
@@ -304,13 +354,11 @@ __global__ __launch_bounds__(kBlock) void k_mask(const FieldDev *__restrict__ fi
 // by (1/(K+2*alpha))^n and, on the last pass, forms the masked FP64 quotient
 // directly from the FP64 window sums of both fields.
 
-constexpr int kSegArr = 41;   // elements per lane segment (odd: conflict-free lane stride)
+// Segment lengths (elements per lane) the fast pass is instantiated for; odd, so the LDS
+// lane stride is conflict-free.  The host picks one per pass and line length.
+#define FB_FAST_SEGS(X) X(25) X(29) X(33) X(37) X(41) X(45) X(49)
 constexpr int kFastBlock = 512;   // threads of a fast-pass workgroup: 256 per field
-#ifndef FB_ADD_BLOCK
-#define FB_ADD_BLOCK 4
-#endif
-constexpr int kBlk = FB_ADD_BLOCK;                    // add-stream block (LDS loads in flight)
-constexpr int kNBlk = (kSegArr + kBlk - 1) / kBlk;    // blocks covering the kSegArr steps
+constexpr int kBlk = 4;           // add-stream block (LDS loads in flight)
 
 // One iteration of the tailed box filter over the lane's segment starting at s of one
 // field's line `lin`.  The subtract stream lin[s-T+j] (j < kSegArr) is read in one burst
@@ -321,10 +369,12 @@ constexpr int kNBlk = (kSegArr + kBlk - 1) / kBlk;    // blocks covering the kSe
 // discarded by stage_store.  Results replace sb[j] (dead after step j: its value lives on
 // in p), so the register array doubles as the staging buffer of the in-place update.
 //   SCALE: multiply the results by `scale` (last iteration of a pass).
-template <bool SCALE>
+template <int SEG, bool SCALE>
 __device__ __forceinline__ void slide1(const float *__restrict__ lin, int s, int T, double alpha, double scale,
-                                       float (&sb)[kSegArr])
+                                       float (&sb)[SEG])
 {
+    constexpr int kSegArr = SEG;
+    constexpr int kNBlk = (SEG + kBlk - 1) / kBlk;   // blocks covering the SEG steps
     const int K = 2 * T + 1;
     const float *subp = lin + s - T;
 #pragma unroll
@@ -385,9 +435,11 @@ __device__ __forceinline__ void slide1(const float *__restrict__ lin, int s, int
 
 // n: elements of the segment inside the line; full: the line's right padding holds at
 // least kSegArr-1 floats, so the whole segment may be stored (the caller re-zeroes the padding)
+template <int SEG>
 __device__ __forceinline__ void stage_store(float *__restrict__ lin, int s, int n, bool full,
-                                            const float (&sb)[kSegArr])
+                                            const float (&sb)[SEG])
 {
+    constexpr int kSegArr = SEG;
     if (full || n == kSegArr) {
 #pragma unroll
         for (int j = 0; j < kSegArr; ++j) lin[s + j] = sb[j];
@@ -487,7 +539,7 @@ __device__ __forceinline__ void tile_copy(float *__restrict__ g, int64_t ls, int
 // One workgroup = kFastBlock threads: the first half works on the v field, the second half
 // on the w field of the same nlines lines (each half: lines_cap = 256/lpl lines of lpl
 // lanes).  LDS holds the v lines, then the w lines, each padded by T+1 zeros per side.
-template <bool INJECT>
+template <bool INJECT, int SEG>
 __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     const FieldDev *__restrict__ fields, const double *__restrict__ offsets, const PassWork *__restrict__ work,
     const PassField *__restrict__ passes, const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
@@ -495,6 +547,7 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     const double *__restrict__ val, float *__restrict__ gv, float *__restrict__ gw, float *__restrict__ out,
     float *__restrict__ wsave, int final_pass)
 {
+    constexpr int kSegArr = SEG;
     extern __shared__ __align__(16) unsigned char smem[];
     const PassWork wk = work[blockIdx.x];
     const FieldDev &F = fields[wk.field];
@@ -520,17 +573,10 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     const bool line_ok = my_line < nl;
     const double off = offsets[wk.field];
 
-    // injection: fetch this line's contribution range before zeroing the LDS
+    // injection: the first (up to 64) lanes of each line
     const int iw = lpl < 64 ? lpl : 64;
     const bool injector = INJECT && line_ok && sid < iw;
-    const int64_t W = F.size[0], H = F.size[1];
-    const int64_t line_in_field = INJECT ? wk.base / W + my_line : 0;
-    uint32_t ls = 0, lc = 0;
-    if (injector) {
-        const int64_t gl = F.xline_off + line_in_field;
-        ls = line_start[gl];
-        lc = line_count[gl];
-    }
+    const int64_t line_in_field = INJECT ? wk.base / F.size[0] + my_line : 0;
 
     if (INJECT) {
         float4 *z = reinterpret_cast<float4 *>(smem);
@@ -552,41 +598,61 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     __syncthreads();
 
     if (INJECT) {
-        // Deterministic injection.  The line's contributions arrive sorted by cell (point
-        // order inside a cell).  The first wave of the line's workers takes them in chunks of
-        // up to 64; a segmented suffix sum over lanes of equal cell (fixed tree order) gives
-        // each run's total to its first lane, which adds it to the cell of its own field.
-        // Chunks of one line are handled by one wave in program order, so the result does
-        // not depend on timing.
-        int nch = injector ? (int)((lc + iw - 1) / iw) : 0;
+        // Deterministic injection.  The line's sub-streams (sub_stream: one base row, one
+        // corner column), concatenated, are taken in chunks of up to 64 entries by the line's
+        // first lanes.  A segmented suffix sum over adjacent lanes of equal target cell (fixed
+        // tree order) gives each run's total to its first lane.  Equal cells adjacent in one
+        // sub-stream form one run; a cell can head two runs of a chunk only from different
+        // sub-streams, so the heads add their totals to LDS in one round per sub-stream of
+        // the head (program order), without conflicts.  The result does not depend on timing.
+        const int nsub = 2 * source_rows(F.dim);
+        uint32_t rsq[8], pre[9];
+        pre[0] = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            uint32_t rs = 0, rc = 0;
+            if (injector && q < nsub) sub_stream(F, line_in_field, q, line_start, line_count, rs, rc);
+            rsq[q] = rs;
+            pre[q + 1] = pre[q] + rc;
+        }
+        int nch = (int)((pre[8] + iw - 1) / iw);
         for (int d = 32; d >= 1; d >>= 1) nch = max(nch, __shfl_xor(nch, d));
-        const int64_t ly = line_in_field % H, lz = line_in_field / H;
         for (int ch = 0; ch < nch; ++ch) {
             const uint32_t e = (uint32_t)ch * iw + sid;
-            const bool valid = injector && e < lc;
-            int x = -1;
+            const bool valid = injector && e < pre[8];
+            int x = -1, hq = 0;
             double c = 0.0;
             if (valid) {
-                const uint32_t key = keys[ls + e];
-                const uint32_t p = vals[ls + e];
-                const int64_t cell = (int64_t)key - F.cell_off;
-                const int64_t xl = cell - line_in_field * W;
-                x = ((uint64_t)xl < (uint64_t)W) ? (int)xl : -1;   // -1 cannot happen for a consistent sort
+                uint32_t src = 0;
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (e >= pre[q] && e < pre[q + 1]) {
+                        hq = q;
+                        src = rsq[q] + (e - pre[q]);
+                    }
+                const uint32_t p = vals[src];
                 PointPos pp;
                 point_pos(F, pts + (int64_t)p * F.dim, pp);
-                const double w = corner_weight(F.dim, pp, x - pp.i[0], ly - pp.i[1], lz - pp.i[2]);
+                x = (int)pp.i[0] + (hq & 1);
+                const double w = corner_weight(F.dim, pp, hq & 1, (hq >> 1) & 1, hq >> 2);
                 c = fsel ? w : w * (val[p] - off);
             }
+            // runs: maximal groups of adjacent lanes with equal cell; run id = inclusive count
+            // of run starts (the concatenated stream is not sorted, so equal cells further
+            // apart belong to different runs)
+            const int px = __shfl_up(x, 1, iw);
+            const bool start = sid == 0 || px != x;
+            const int lane = threadIdx.x & 63;
+            const int rid = __popcll(__ballot(start) & ((2ull << lane) - 1ull));
             for (int d = 1; d < iw; d <<= 1) {
                 const double oc = __shfl_down(c, d, iw);
-                const int ox = __shfl_down(x, d, iw);
-                if (sid + d < iw && ox == x) c += oc;
+                const int orid = __shfl_down(rid, d, iw);
+                if (sid + d < iw && orid == rid) c += oc;
             }
-            const int px = __shfl_up(x, 1, iw);
-            if (valid && x >= 0 && (sid == 0 || px != x)) {
-                float *cell = mine + my_line * lstride + pad + x;
-                *cell = (float)((double)*cell + c);
-            }
+            const bool head = valid && start;
+            float *cell = mine + my_line * lstride + pad + (x < 0 ? 0 : x);
+            for (int q = 0; q < nsub; ++q)
+                if (head && hq == q) *cell = (float)((double)*cell + c);
         }
         __syncthreads();
     }
@@ -613,8 +679,8 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
         const bool last = it + 1 == niter;
         const bool grp = (it + 1) % P.sgroup == 0;
         if (act) {
-            if (last || grp) slide1<true>(ln, s0, T, alpha, last ? P.slast : P.sgscale, sb);
-            else slide1<false>(ln, s0, T, alpha, 1.0, sb);
+            if (last || grp) slide1<SEG, true>(ln, s0, T, alpha, last ? P.slast : P.sgscale, sb);
+            else slide1<SEG, false>(ln, s0, T, alpha, 1.0, sb);
         }
         __syncthreads();
         if (act) stage_store(ln, s0, myseg, full, sb);
@@ -842,25 +908,24 @@ int launch_offsets(const FieldDev *d_fields, int nfields, const double *d_val, d
     return FB_OK;
 }
 
-int launch_contrib(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints, int corners,
-                   const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_line_count, hipStream_t s)
+int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
+                      const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_row_count, hipStream_t s)
 {
     if (npoints <= 0) return FB_OK;
-    hipLaunchKernelGGL(k_contrib, dim3(nblocks(npoints)), dim3(kBlock), 0, s, d_fields, d_pt_prefix, nfields,
-                       npoints, corners, d_pts, d_keys, d_vals, d_line_count);
+    hipLaunchKernelGGL(k_point_keys, dim3(nblocks(npoints)), dim3(kBlock), 0, s, d_fields, d_pt_prefix, nfields,
+                       npoints, d_pts, d_keys, d_vals, d_row_count);
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }
 
 int launch_exact_inject(const FieldDev *d_fields, const int64_t *d_xline_prefix, int nfields, int64_t nxlines,
-                        const uint32_t *d_keys, const uint32_t *d_vals, const uint32_t *d_line_start,
-                        const uint32_t *d_line_count, const double *d_pts, const double *d_val,
-                        const double *d_offsets, double *d_vg, double *d_wg, hipStream_t s)
+                        const uint32_t *d_vals, const uint32_t *d_row_start, const uint32_t *d_row_count,
+                        const double *d_pts, const double *d_val, const double *d_offsets, double *d_vg,
+                        double *d_wg, hipStream_t s)
 {
     if (nxlines <= 0) return FB_OK;
     hipLaunchKernelGGL(k_exact_inject, dim3(nblocks(nxlines)), dim3(kBlock), 0, s, d_fields, d_xline_prefix,
-                       nfields, nxlines, d_keys, d_vals, d_line_start, d_line_count, d_pts, d_val, d_offsets, d_vg,
-                       d_wg);
+                       nfields, nxlines, d_vals, d_row_start, d_row_count, d_pts, d_val, d_offsets, d_vg, d_wg);
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }
@@ -905,11 +970,15 @@ int fast_pass_init()
     static std::once_flag once;
     static hipError_t err = hipSuccess;
     std::call_once(once, [] {
-        err = hipFuncSetAttribute((const void *)k_fast_pass<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  fast_pass_max_lds());
-        if (err == hipSuccess)
-            err = hipFuncSetAttribute((const void *)k_fast_pass<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      fast_pass_max_lds());
+        const void *fns[] = {
+#define FB_FN(S) (const void *)k_fast_pass<true, S>, (const void *)k_fast_pass<false, S>,
+            FB_FAST_SEGS(FB_FN)
+#undef FB_FN
+        };
+        for (const void *fn : fns) {
+            err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, fast_pass_max_lds());
+            if (err != hipSuccess) break;
+        }
     });
     if (err != hipSuccess) {
         (void)hipGetLastError();
@@ -919,7 +988,19 @@ int fast_pass_init()
     return FB_OK;
 }
 
-int launch_fast_pass(bool inject, const FieldDev *d_fields, const double *d_offsets, const PassWork *d_work,
+bool fast_pass_seg_supported(int seg)
+{
+    switch (seg) {
+#define FB_CASE(S) case S:
+        FB_FAST_SEGS(FB_CASE)
+#undef FB_CASE
+        return true;
+    default:
+        return false;
+    }
+}
+
+int launch_fast_pass(bool inject, int seg, const FieldDev *d_fields, const double *d_offsets, const PassWork *d_work,
                      int nwork, const PassField *d_pass, int threads, size_t lds_bytes, const uint32_t *d_keys,
                      const uint32_t *d_vals, const uint32_t *d_line_start, const uint32_t *d_line_count,
                      const double *d_pts, const double *d_val, float *d_v, float *d_w, float *d_out, float *d_wsave,
@@ -930,14 +1011,23 @@ int launch_fast_pass(bool inject, const FieldDev *d_fields, const double *d_offs
         set_error("fast pass: unsupported workgroup size");
         return FB_ERR_UNSUPPORTED;
     }
-    if (inject) {
-        hipLaunchKernelGGL(k_fast_pass<true>, dim3(nwork), dim3(threads), lds_bytes, s, d_fields, d_offsets, d_work,
-                           d_pass, d_keys, d_vals, d_line_start, d_line_count, d_pts, d_val, d_v, d_w, d_out,
-                           d_wsave, final_pass);
-    } else {
-        hipLaunchKernelGGL(k_fast_pass<false>, dim3(nwork), dim3(threads), lds_bytes, s, d_fields, d_offsets,
-                           d_work, d_pass, d_keys, d_vals, d_line_start, d_line_count, d_pts, d_val, d_v, d_w, d_out,
-                           d_wsave, final_pass);
+    switch (seg) {
+#define FB_LAUNCH(S)                                                                                               \
+    case S:                                                                                                        \
+        if (inject)                                                                                                \
+            hipLaunchKernelGGL((k_fast_pass<true, S>), dim3(nwork), dim3(threads), lds_bytes, s, d_fields,         \
+                               d_offsets, d_work, d_pass, d_keys, d_vals, d_line_start, d_line_count, d_pts, d_val, \
+                               d_v, d_w, d_out, d_wsave, final_pass);                                              \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_fast_pass<false, S>), dim3(nwork), dim3(threads), lds_bytes, s, d_fields,        \
+                               d_offsets, d_work, d_pass, d_keys, d_vals, d_line_start, d_line_count, d_pts, d_val, \
+                               d_v, d_w, d_out, d_wsave, final_pass);                                              \
+        break;
+        FB_FAST_SEGS(FB_LAUNCH)
+#undef FB_LAUNCH
+    default:
+        set_error("fast pass: unsupported segment length " + std::to_string(seg));
+        return FB_ERR_UNSUPPORTED;
     }
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;

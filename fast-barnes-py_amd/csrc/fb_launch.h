@@ -17,15 +17,16 @@ int radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_
 // fb_kernels.hip -- injection front end
 int launch_offsets(const FieldDev *d_fields, int nfields, const double *d_val, double *d_offsets,
                    hipStream_t stream);
-int launch_contrib(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
-                   int corners, const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_line_count,
-                   hipStream_t stream);
+// one (base cell, point) pair per point; per-base-row point counts (row = x-line index)
+int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
+                      const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_row_count,
+                      hipStream_t stream);
 
 // fb_kernels.hip -- FP64 path with the reference's operation order
 int launch_exact_inject(const FieldDev *d_fields, const int64_t *d_xline_prefix, int nfields, int64_t nxlines,
-                        const uint32_t *d_keys, const uint32_t *d_vals, const uint32_t *d_line_start,
-                        const uint32_t *d_line_count, const double *d_pts, const double *d_val,
-                        const double *d_offsets, double *d_vg, double *d_wg, hipStream_t stream);
+                        const uint32_t *d_vals, const uint32_t *d_row_start, const uint32_t *d_row_count,
+                        const double *d_pts, const double *d_val, const double *d_offsets, double *d_vg,
+                        double *d_wg, hipStream_t stream);
 // One thread per (line, field component); lines of axis `axis` of every field.
 // line_prefix[f] = first (2*line) slot of field f; total = line_prefix[nfields].
 // rect_len < 0 selects 2*T[axis]+1 from the field.
@@ -41,7 +42,8 @@ int launch_mask(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfie
 // fb_kernels.hip -- FP32C path
 int fast_pass_max_lds();
 int fast_pass_init();
-int launch_fast_pass(bool inject, const FieldDev *d_fields, const double *d_offsets, const PassWork *d_work,
+bool fast_pass_seg_supported(int seg);
+int launch_fast_pass(bool inject, int seg, const FieldDev *d_fields, const double *d_offsets, const PassWork *d_work,
                      int nwork, const PassField *d_pass, int threads, size_t lds_bytes, const uint32_t *d_keys,
                      const uint32_t *d_vals, const uint32_t *d_line_start, const uint32_t *d_line_count,
                      const double *d_pts, const double *d_val, float *d_v, float *d_w, float *d_out,
@@ -54,6 +56,13 @@ int launch_adj_prep(const FieldDev *d_fields, const int64_t *d_cell_prefix, int 
 int launch_adj_gather(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
                       const double *d_pts, const double *d_val, const double *d_offsets, const float *d_A,
                       const float *d_Bb, double *d_gval, double *d_gpts, hipStream_t stream);
+
+// fb_s2.hip -- spherical variant (proj = center_lon, n, n_inv, F, rho0)
+int launch_lambert_points(const double *d_pts, double *d_out, int64_t n, const double *proj, hipStream_t stream);
+int launch_resample(const float *d_lam, const int64_t *lam_size, const double *lam_x0, const double *x0,
+                    const double *step, const int64_t *size, const double *proj, float *d_out, hipStream_t stream);
+int launch_naive_s2(const double *d_pts, const double *d_val, const double *d_offset, int64_t n, double sigma,
+                    const double *x0, const double *step, const int64_t *size, double *d_out, hipStream_t stream);
 
 // fb_kernels.hip -- elementwise helpers
 int launch_zero(void *p, int64_t bytes, hipStream_t stream);   // bytes: multiple of 4

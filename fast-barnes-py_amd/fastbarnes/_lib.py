@@ -66,6 +66,8 @@ SIGNATURES = {
                                     _dp]),
     'fb_barnes': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _vp, _vp, _dp, _dp, _dp,
                                  _i64p, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, ctypes.c_uint, _vp]),
+    'fb_barnes_s2': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, _vp, _vp, _dp, _dp, _dp, _i64p,
+                                    ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, ctypes.c_uint, _vp]),
     'fb_accumulate_lines': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
                                            ctypes.c_int, ctypes.c_double, _vp, _vp, ctypes.c_uint, _vp]),
     'fb_convolve_grid': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i64p, _i32p, ctypes.c_int, _dp,
@@ -76,6 +78,8 @@ SIGNATURES = {
     'fb_plan_execute_graph': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     'fb_plan_forward': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     'fb_plan_backward': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'fb_plan_autotune': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    'fb_plan_pass_groups': (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     'fb_plan_set_timing': (ctypes.c_int, [_vp, ctypes.c_int]),
     'fb_plan_stage_times': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_int]),

@@ -117,6 +117,19 @@ class BarnesBatch:
         check(lib().fb_plan_backward(self._plan, d_pts, d_val, d_out, d_wsave, d_gout, d_gval, d_gpts or None,
                                      stream))
 
+    def autotune(self, d_pts, d_val, d_out, stream):
+        """Times the segment-length variants of every line pass on these buffers and keeps
+        the fastest per axis and line length (FP32C plans).  d_out is left undefined."""
+        check(lib().fb_plan_autotune(self._plan, d_pts, d_val, d_out, stream))
+
+    def pass_groups(self, axis):
+        """[(segment length, work items)] of the launches of the pass along `axis`."""
+        seg = (ctypes.c_int32 * 16)()
+        items = (ctypes.c_int32 * 16)()
+        n = ctypes.c_int()
+        check(lib().fb_plan_pass_groups(self._plan, axis, seg, items, 16, ctypes.byref(n)))
+        return [(seg[i], items[i]) for i in range(min(n.value, 16))]
+
     def set_timing(self, enable):
         check(lib().fb_plan_set_timing(self._plan, int(bool(enable))))
 

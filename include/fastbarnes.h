@@ -88,6 +88,21 @@ int fb_barnes(int method, int precision, int dim, int64_t n, const double *pts, 
               int num_iter, double max_dist_weight, double min_weight, void *out, unsigned flags,
               void *stream);
 
+/* barnes_S2(pts, val, sigma, x0, step, size, method, num_iter, max_dist, resample)
+ *   interpolationS2.py:31-154 (dispatch :146-154):
+ *   FB_METHOD_OPTIMIZED_CONVOLUTION = 'optimized_convolution_S2' :157-219 -- Lambert conformal
+ *     projection (util/lambert_conformal.py:115-125, projection create_proj(11.5, 34.5,
+ *     42.5, 65.5) :50-97) onto the fixed grid x0 = (-32, -2), size (int(64/step_x),
+ *     int(44/step_y)), the optimized convolution there, then _resample :221-267 to the
+ *     lon/lat grid.  out: float32, size[1] x size[0] if resample, else the Lambert field
+ *     (float32, int(44/step_y) x int(64/step_x)).
+ *   FB_METHOD_NAIVE = 'naive_S2' :272-320 -- spherical distance; out float64 size[1] x size[0].
+ * pts rows are (lon, lat) in degrees; sigma, x0, step, size have 2 entries.  flags and
+ * stream as for fb_barnes. */
+int fb_barnes_s2(int method, int precision, int64_t n, const double *pts, const double *val,
+                 const double *sigma, const double *x0, const double *step, const int64_t *size,
+                 int num_iter, double max_dist_weight, int resample, void *out, unsigned flags, void *stream);
+
 /* ---- line and grid primitives behind the reference's private helpers ---- */
 
 /* num_iter-fold box filter of `nlines` lines of length len (row-major nlines x len, FP64):
@@ -155,6 +170,16 @@ int fb_plan_forward(fb_plan *plan, const double *d_pts, const double *d_val, flo
  * stream-ordered, not concurrent.  No counterpart in the reference. */
 int fb_plan_backward(fb_plan *plan, const double *d_pts, const double *d_val, const float *d_out,
                      const float *d_wsave, const float *d_gout, double *d_gval, double *d_gpts, void *stream);
+
+/* Autotuner of the FP32C line passes (no reference counterpart: the reference has no
+ * GPU tiling).  For every axis and line length of the plan it times each segment length
+ * (elements per lane; it sets the lanes per line and the lines per workgroup) on the plan's
+ * own buffers and keeps the fastest; untuned plans use a cost model.  Leaves d_out
+ * undefined: execute again afterwards.  A no-op for FP64 plans. */
+int fb_plan_autotune(fb_plan *plan, const double *d_pts, const double *d_val, float *d_out, void *stream);
+
+/* The launches of the fast pass along `axis`: segment length and work items of each. */
+int fb_plan_pass_groups(fb_plan *plan, int axis, int32_t *seg, int32_t *items, int max_groups, int *ngroups);
 
 /* Per-stage device time (ms) of the last fb_plan_execute run with timing enabled,
  * measured with HIP events recorded on the execution stream.  names receives a

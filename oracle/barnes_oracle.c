@@ -560,3 +560,138 @@ void orc_adjoint(int tail, int dim, const double *pts, const double *val_in, int
     (void)W; (void)H;
     free(tmp); free(val); free(vg); free(wg); free(fout);
 }
+
+/* ---- spherical variant ---------------------------------------------------
+ * fastbarnes/util/lambert_conformal.py and fastbarnes/interpolationS2.py. */
+
+static const double ORC_RAD = 3.141592653589793 / 180.0;
+
+/* create_proj, lambert_conformal.py:50-97: proj = (center_lon, n, n_inv, F, rho0) */
+void orc_lambert_create(double center_lon, double center_lat, double lat1, double lat2, double *proj)
+{
+    const double h = ORC_RAD / 2.0;
+    double n;
+    if (lat1 != lat2)
+        n = log(cos(lat1 * ORC_RAD) / cos(lat2 * ORC_RAD)) / log(tan((90.0 + lat2) * h) / tan((90.0 + lat1) * h));
+    else
+        n = sin(lat1 * ORC_RAD);
+    const double F = cos(lat1 * ORC_RAD) * pow(tan((90.0 + lat1) * h), n) / n;
+    const double rho0 = F / pow(tan((90.0 + center_lat) * h), n);
+    proj[0] = center_lon; proj[1] = n; proj[2] = 1.0 / n; proj[3] = F; proj[4] = rho0;
+}
+
+/* get_scale, lambert_conformal.py:102-110 */
+double orc_lambert_scale(double lat, const double *proj)
+{
+    return proj[3] * proj[1] / (cos(lat * ORC_RAD) * pow(tan((90.0 + lat) * (ORC_RAD / 2.0)), proj[1]));
+}
+
+/* to_map, lambert_conformal.py:115-125 (geo rows are (lon, lat)) */
+void orc_lambert_to_map(const double *geo, double *map, int64_t n, const double *proj)
+{
+    for (int64_t k = 0; k < n; k++) {
+        const double rho = proj[3] / pow(tan((90.0 + geo[2 * k + 1]) * (ORC_RAD / 2.0)), proj[1]);
+        const double arg = proj[1] * (geo[2 * k] - proj[0]) * ORC_RAD;
+        map[2 * k] = rho * sin(arg) / ORC_RAD;
+        map[2 * k + 1] = (proj[4] - rho * cos(arg)) / ORC_RAD;
+    }
+}
+
+/* to_geo, lambert_conformal.py:143-160 */
+void orc_lambert_to_geo(const double *map, double *geo, int64_t n, const double *proj)
+{
+    for (int64_t k = 0; k < n; k++) {
+        const double x = map[2 * k] * ORC_RAD, y = map[2 * k + 1] * ORC_RAD;
+        const double arg = proj[4] - y;
+        double rho = sqrt(x * x + arg * arg);
+        if (proj[1] < 0.0) rho = -rho;
+        const double theta = atan2(x, arg);
+        geo[2 * k + 1] = atan(pow(proj[3] / rho, proj[2])) / (ORC_RAD / 2.0) - 90.0;
+        geo[2 * k] = proj[0] + theta / proj[1] / ORC_RAD;
+    }
+}
+
+/* _resample, interpolationS2.py:221-267.  Nodes whose 2x2 Lambert neighbourhood leaves the
+ * Lambert grid are NaN here (the reference does not check the bounds). */
+void orc_resample(const float *lam, const double *lam_x0, const int64_t *lam_size, const double *x0,
+                  const double *step, const int64_t *size, const double *proj, float *out)
+{
+    for (int64_t j = 0; j < size[1]; j++) {
+        const double geoy = (double)j * step[1] + x0[1];
+        const double rho = proj[3] / pow(tan((90.0 + geoy) * (ORC_RAD / 2.0)), proj[1]);
+        for (int64_t i = 0; i < size[0]; i++) {
+            const double geox = x0[0] + (double)i * step[0];
+            const double arg = proj[1] * (geox - proj[0]) * ORC_RAD;
+            double mx = rho * sin(arg) / ORC_RAD;
+            double my = (proj[4] - rho * cos(arg)) / ORC_RAD;
+            mx = (mx - lam_x0[0]) / step[0];
+            my = (my - lam_x0[1]) / step[1];
+            const int32_t ix = (int32_t)mx, iy = (int32_t)my;
+            mx -= (double)ix;
+            my -= (double)iy;
+            if (!(ix >= 0 && iy >= 0 && ix + 1 < lam_size[0] && iy + 1 < lam_size[1])) {
+                out[j * size[0] + i] = NAN;
+                continue;
+            }
+            const int64_t W = lam_size[0];
+            out[j * size[0] + i] = (float)((1.0 - my) * (1.0 - mx) * (double)lam[iy * W + ix] +
+                                           my * (1.0 - mx) * (double)lam[(iy + 1) * W + ix] +
+                                           my * mx * (double)lam[(iy + 1) * W + ix + 1] +
+                                           (1.0 - my) * mx * (double)lam[iy * W + ix + 1]);
+        }
+    }
+}
+
+/* _interpolate_opt_convol_S2 with its two parts, interpolationS2.py:157-206: project the
+ * points onto the fixed Lambert grid (lam_x0 = (-32, -2), size (64/step, 44/step)), run the
+ * optimized convolution there, resample to the lon/lat grid (resample != 0) or return the
+ * Lambert field (out sized lam_size[1] x lam_size[0]). */
+void orc_barnes_s2_opt(const double *pts, const double *val, int64_t n, const double *sigma, const double *x0,
+                       const double *step, const int64_t *size, int64_t num_iter, double max_dist_weight,
+                       int resample, float *out)
+{
+    double proj[5];
+    orc_lambert_create(11.5, 34.5, 42.5, 65.5, proj);
+    const double lam_x0[2] = {-32.0, -2.0};
+    const int64_t lam_size[2] = {(int64_t)(64.0 / step[0]), (int64_t)(44.0 / step[1])};
+    double *lam_pts = (double *)malloc(sizeof(double) * 2 * (n > 0 ? n : 1));
+    orc_lambert_to_map(pts, lam_pts, n, proj);
+    float *lam = resample ? (float *)malloc(sizeof(float) * lam_size[0] * lam_size[1]) : out;
+    orc_interpolate_convol(1, 2, lam_pts, val, n, sigma, lam_x0, step, lam_size, num_iter, max_dist_weight, lam,
+                           NULL, NULL);
+    if (resample) {
+        orc_resample(lam, lam_x0, lam_size, x0, step, size, proj, out);
+        free(lam);
+    }
+    free(lam_pts);
+}
+
+/* _interpolate_naive_S2 and _dist_S2, interpolationS2.py:272-320: spherical distance in
+ * degrees, sequential sums (the reference's np.dot may associate differently). */
+void orc_naive_s2(const double *pts, const double *val_in, int64_t n, const double *sigma, const double *x0,
+                  const double *step, const int64_t *size, double *out)
+{
+    double *val = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    memcpy(val, val_in, sizeof(double) * n);
+    const double offset = orc_normalize_values(val, n);
+    const double scale = 2 * sq(sigma[0]);
+    for (int64_t j = 0; j < size[1]; j++) {
+        const double yc = x0[1] + (double)j * step[1];
+        const double lat0 = yc * ORC_RAD;
+        for (int64_t i = 0; i < size[0]; i++) {
+            const double xc = x0[0] + (double)i * step[0];
+            double ws = 0.0, wt = 0.0;
+            for (int64_t k = 0; k < n; k++) {
+                const double lat1 = pts[2 * k + 1] * ORC_RAD;
+                double arg = sin(lat0) * sin(lat1) + cos(lat0) * cos(lat1) * cos((pts[2 * k] - xc) * ORC_RAD);
+                if (arg > 1.0) arg = 1.0;
+                const double d = acos(arg) / ORC_RAD;
+                const double w = exp(-d * d / scale);
+                ws += w * val[k];
+                wt += w;
+            }
+            out[j * size[0] + i] = wt > 0.0 ? ws / wt + offset : NAN;
+        }
+    }
+    free(val);
+}

@@ -65,6 +65,15 @@ def lib():
         L.orc_filter_grid.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _i64p, ctypes.c_int64]
         L.orc_adjoint.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp, _dp, _dp, _i64p,
                                   ctypes.c_int64, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_lambert_create.argtypes = [ctypes.c_double] * 4 + [_dp]
+        L.orc_lambert_scale.argtypes = [ctypes.c_double, _dp]
+        L.orc_lambert_scale.restype = ctypes.c_double
+        L.orc_lambert_to_map.argtypes = [_dp, _dp, ctypes.c_int64, _dp]
+        L.orc_lambert_to_geo.argtypes = [_dp, _dp, ctypes.c_int64, _dp]
+        L.orc_resample.argtypes = [_fp, _dp, _i64p, _dp, _dp, _i64p, _dp, _fp]
+        L.orc_barnes_s2_opt.argtypes = [_dp, _dp, ctypes.c_int64, _dp, _dp, _dp, _i64p, ctypes.c_int64,
+                                        ctypes.c_double, ctypes.c_int, _fp]
+        L.orc_naive_s2.argtypes = [_dp, _dp, ctypes.c_int64, _dp, _dp, _dp, _i64p, _dp]
         _lib = L
     return _lib
 
@@ -237,3 +246,69 @@ def adjoint(pts, val, sigma, x0, step, size, g, num_iter=4, max_dist=3.5, tail=T
     lib().orc_adjoint(int(tail), dim, _d(pts), _d(val), len(val), _d(sg), _d(x0v), _d(st),
                       sz.ctypes.data_as(_i64p), num_iter, mdw, _d(gg), _d(gval), _d(gpts))
     return gval, gpts
+
+
+# ---- spherical variant (interpolationS2.py, util/lambert_conformal.py) ----------
+
+def lambert_proj(center_lon=11.5, center_lat=34.5, lat1=42.5, lat2=65.5):
+    p = np.empty(5)
+    lib().orc_lambert_create(center_lon, center_lat, lat1, lat2, _d(p))
+    return p
+
+
+def lambert_scale(lat, proj):
+    return lib().orc_lambert_scale(float(lat), _d(np.ascontiguousarray(proj, dtype=np.float64)))
+
+
+def lambert_to_map(geo, proj):
+    g = np.ascontiguousarray(np.asarray(geo, dtype=np.float64).reshape(-1, 2))
+    m = np.empty_like(g)
+    lib().orc_lambert_to_map(_d(g), _d(m), len(g), _d(np.ascontiguousarray(proj, dtype=np.float64)))
+    return m
+
+
+def lambert_to_geo(mp, proj):
+    m = np.ascontiguousarray(np.asarray(mp, dtype=np.float64).reshape(-1, 2))
+    g = np.empty_like(m)
+    lib().orc_lambert_to_geo(_d(m), _d(g), len(m), _d(np.ascontiguousarray(proj, dtype=np.float64)))
+    return g
+
+
+def lambert_grid(step):
+    """(lam_x0, lam_size) of the fixed Lambert grid of barnes_S2 (interpolationS2.py:186-188)."""
+    st = _vec(step, 2)
+    return np.array([-32.0, -2.0]), (int(64.0 / st[0]), int(44.0 / st[1]))
+
+
+def barnes_s2_opt(pts, val, sigma, x0, step, size, num_iter=4, max_dist=3.5, resample=True):
+    pts, dim, val, sg, x0v, st, sz = _common(pts, val, sigma, x0, step, size)
+    assert dim == 2
+    mdw = math.exp(-max_dist ** 2 / 2)
+    if resample:
+        out = np.empty((int(sz[1]), int(sz[0])), dtype=np.float32)
+    else:
+        lam_size = lambert_grid(st)[1]
+        out = np.empty((lam_size[1], lam_size[0]), dtype=np.float32)
+    lib().orc_barnes_s2_opt(_d(pts), _d(val), len(val), _d(sg), _d(x0v), _d(st), sz.ctypes.data_as(_i64p),
+                            num_iter, mdw, int(bool(resample)), out.ctypes.data_as(_fp))
+    return out
+
+
+def resample(lam_field, step, x0, size, proj=None):
+    lam = np.ascontiguousarray(lam_field, dtype=np.float32)
+    st, x0v = _vec(step, 2), _vec(x0, 2)
+    lam_x0 = np.array([-32.0, -2.0])
+    lsz = np.asarray(lam.shape[::-1], dtype=np.int64)
+    sz = np.asarray(size, dtype=np.int64)
+    out = np.empty((int(sz[1]), int(sz[0])), dtype=np.float32)
+    p = lambert_proj() if proj is None else np.ascontiguousarray(proj, dtype=np.float64)
+    lib().orc_resample(lam.ctypes.data_as(_fp), _d(lam_x0), lsz.ctypes.data_as(_i64p), _d(x0v), _d(st),
+                       sz.ctypes.data_as(_i64p), _d(p), out.ctypes.data_as(_fp))
+    return out
+
+
+def naive_s2(pts, val, sigma, x0, step, size):
+    pts, dim, val, sg, x0v, st, sz = _common(pts, val, sigma, x0, step, size)
+    out = np.empty((int(sz[1]), int(sz[0])), dtype=np.float64)
+    lib().orc_naive_s2(_d(pts), _d(val), len(val), _d(sg), _d(x0v), _d(st), sz.ctypes.data_as(_i64p), _d(out))
+    return out

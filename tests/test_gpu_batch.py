@@ -107,3 +107,36 @@ def test_batch_3d_fp64_equals_single():
             ref = barnes(c['pts'], c['val'], c['sigma'], c['x0'], c['step'], c['size'], num_iter=c['num_iter'],
                          max_dist=c['max_dist'], precision=prec)
             _bitwise(o, ref)
+
+
+def test_autotune_keeps_results_and_reports_groups():
+    """The autotuner times the segment-length variants of every line pass on the plan's own
+    buffers and rebuilds the passes; results stay within FP32C rounding of the untuned plan
+    (different segment anchors change only the last bits), execution stays deterministic and
+    graph replays are rebuilt."""
+    from fastbarnes import hip, workloads
+    from fastbarnes.batched import BarnesBatch
+    fs = [workloads.config3_field(i) for i in range(6)] + [workloads.config4(seed=7, n=3000, size=(64, 48, 24))]
+    for dim_group in ([c for c in fs if len(c['size']) == 2], [c for c in fs if len(c['size']) == 3]):
+        batch = BarnesBatch([_desc(c) for c in dim_group], precision='fp32c')
+        pts = np.concatenate([c['pts'] for c in dim_group])
+        val = np.concatenate([c['val'] for c in dim_group])
+        d_pts, d_val = hip.DeviceArray.from_numpy(pts), hip.DeviceArray.from_numpy(val)
+        d_out = hip.DeviceArray((batch.total_out,), np.float32)
+        s = hip.Stream()
+        batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr)
+        s.synchronize()
+        before = d_out.numpy()
+        batch.autotune(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr)
+        for axis in range(len(dim_group[0]['size'])):
+            groups = batch.pass_groups(axis)
+            assert groups and all(seg in (25, 29, 33, 37, 41, 45, 49) and n > 0 for seg, n in groups)
+        batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr, graph=True)
+        s.synchronize()
+        after = d_out.numpy()
+        np.testing.assert_array_equal(np.isnan(before), np.isnan(after))
+        ok = ~np.isnan(before)
+        assert np.max(np.abs(after[ok] - before[ok]) / (np.abs(before[ok]) + 1.0)) < 1e-5
+        batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr, graph=False)
+        s.synchronize()
+        _bitwise(d_out.numpy(), after)
