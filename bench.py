@@ -170,6 +170,8 @@ def leg_train(cfgs, steps, warmup):
     d_gval = hip.DeviceArray((batch.total_points,), np.float64)
     d_gpts = hip.DeviceArray(pts.shape, np.float64)
     s = hip.Stream()
+    if AUTOTUNE:
+        batch.autotune(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr)
     batch.forward(d_pts.ptr, d_val.ptr, d_out.ptr, d_ws.ptr, s.ptr)
     s.synchronize()
     out = d_out.numpy()

@@ -22,10 +22,10 @@ const char *last_error() { return g_err.c_str(); }
 
 namespace {
 
-constexpr int kBlock = 256;             // lanes per field of a fast-pass workgroup (the
-                                         // workgroup holds kBlock v workers + kBlock w workers)
-constexpr int kFastThreads = 2 * kBlock;
-constexpr size_t kLdsBudget = 80 * 1024; // two workgroups per CU
+constexpr int kBlock = 256;             // largest number of lanes per field of a fast-pass
+                                         // workgroup (it holds as many v workers as w workers)
+constexpr size_t kLdsBudget = 80 * 1024; // LDS of a 2*kBlock-thread workgroup: two per CU
+constexpr int kLpfs[] = {64, 128, 256};  // lanes per field the passes may run with
 constexpr size_t kLdsMax = 160 * 1024;
 
 int fail(int code, const std::string &msg)
@@ -205,21 +205,24 @@ void scale_groups(double g, int n, double total, PassField &p)
     p.slast = ngroups > 1 ? std::pow(1.0 / g, (double)last) : total;
 }
 
-// lanes per line / lines per workgroup of a fast line pass with `seg` elements per lane
-int pass_geometry(int64_t L, int T, int seg, int &lpl, int &lines, int &rpad, size_t &lds)
+// lanes per line / lines per workgroup of a fast line pass with `seg` elements per lane and
+// workgroups of 2*lpf threads (lpf lanes per field); the LDS budget scales with the
+// workgroup so that every size keeps 16 waves per CU
+int pass_geometry(int64_t L, int T, int seg, int lpf, int &lpl, int &lines, int &rpad, size_t &lds)
 {
     const int64_t need = (L + seg - 1) / seg;
     lpl = 1;
     while (lpl < need) lpl <<= 1;
-    if (lpl > kBlock)
+    if (lpl > lpf)
         return fail(FB_ERR_UNSUPPORTED, "line length " + std::to_string(L) + " exceeds the FP32C limit of " +
-                                            std::to_string(kBlock * seg));
+                                            std::to_string((int64_t)lpf * seg) + " for this workgroup size");
     // branch-free slides read up to seg-1 floats past the last w line: reserve 64 floats
     const size_t reserve = 64 * sizeof(float);
-    const int lane_lines = kBlock / lpl;
+    const int lane_lines = lpf / lpl;
+    const size_t budget = kLdsBudget / (kBlock / lpf);
     auto fit = [&](int rp) {
         const size_t per_line = (size_t)(L + (int64_t)T + 1 + rp) * 2 * sizeof(float);
-        int64_t n = (int64_t)((kLdsBudget - reserve) / per_line);
+        int64_t n = (int64_t)((budget - reserve) / per_line);
         n = std::min<int64_t>(n, lane_lines);
         if (n >= 16) n &= ~(int64_t)3;   // float4 column tiles, at a loss below 1/4
         return (int)std::max<int64_t>(n, 1);
@@ -254,32 +257,51 @@ constexpr int kSegs[] = {25, 29, 33, 37, 41, 45, 49};
 // instructions, about 7 per element of its segment (slide), 2 per window term of the
 // re-anchoring (2T+1 terms) and a fixed ~30 per iteration, plus 2 per element when the
 // last segment's stores are predicated; a workgroup holds `lines` lines.
-double pass_cost(int64_t L, int T, int seg, bool *ok = nullptr)
+double pass_cost(int64_t L, int T, int seg, int lpf = kBlock, bool *ok = nullptr)
 {
     int lpl, lines, rpad;
     size_t lds;
     const std::string keep = last_error();
-    const int rc = pass_geometry(L, T, seg, lpl, lines, rpad, lds);
+    const int rc = pass_geometry(L, T, seg, lpf, lpl, lines, rpad, lds);
     set_error(keep);
     if (ok) *ok = rc == FB_OK;
     if (rc) return 1e300;
     const int over = (int)((seg - L % seg) % seg);
     const bool full = L > seg && rpad >= over;
-    return (7.0 * seg + 2.0 * (2.0 * T + 1.0) + 30.0 + (full ? 0.0 : 2.0 * seg)) / lines;
+    return (7.0 * seg + 2.0 * (2.0 * T + 1.0) + 30.0 + (full ? 0.0 : 2.0 * seg)) / lines * (kBlock / lpf);
 }
 
-int model_seg(int64_t L, int T)
+int model_seg(int64_t L, int T, int lpf = kBlock)
 {
     int best = 41;
     double bc = 1e300;
     for (int seg : kSegs) {
-        const double c = pass_cost(L, T, seg);
+        const double c = pass_cost(L, T, seg, lpf);
         if (c < bc) {
             bc = c;
             best = seg;
         }
     }
     return best;
+}
+
+// segment length and lanes per field of a pass (autotuned or from the cost model)
+struct PassChoice {
+    int seg;
+    int lpf;
+};
+
+// Untuned default: 256-thread workgroups (faster than 512 on every measured workload) when
+// a segment length fits the line in 128 lanes, else 512-thread workgroups.
+PassChoice model_choice(int64_t L, int T)
+{
+    for (int lpf : {kBlock / 2, kBlock}) {
+        const int seg = model_seg(L, T, lpf);
+        bool ok = false;
+        pass_cost(L, T, seg, lpf, &ok);
+        if (ok) return PassChoice{seg, lpf};
+    }
+    return PassChoice{41, kBlock};   // pass_geometry reports the limit
 }
 
 // order work items so that blockIdx b runs tile t with consecutive tiles on one XCD
@@ -316,10 +338,12 @@ namespace {
 // one launch of a fast pass: work items [first, first + count) of the axis's list
 struct PassGroup {
     int seg;
+    int lpf;      // lanes per field: the workgroup has 2*lpf threads
     int first;
     int count;
     size_t lds;
 };
+
 }  // namespace
 }  // namespace fb
 
@@ -345,7 +369,7 @@ struct fb_plan {
     PassWork *d_work[3] = {nullptr, nullptr, nullptr};
     PassField *d_pass[3] = {nullptr, nullptr, nullptr};
     std::vector<PassGroup> groups[3];          // launches of each axis pass, one per segment length
-    std::map<std::pair<int, int64_t>, int> seg_choice;   // (axis, line length) -> segment (autotuned)
+    std::map<std::pair<int, int64_t>, PassChoice> seg_choice;   // (axis, line length) -> autotuned choice
     const uint32_t *sorted_vals = nullptr;      // sorted point indices of the last run
 
     int timing = 0;
@@ -454,12 +478,14 @@ int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<Pa
         const int64_t W = F.size[0], H = F.size[1], D = F.size[2];
         const int64_t L = F.size[axis];
         auto it = P->seg_choice.find({axis, L});
-        const int seg = it != P->seg_choice.end() ? it->second : model_seg(L, F.T[axis]);
+        const PassChoice ch = it != P->seg_choice.end() ? it->second : model_choice(L, F.T[axis]);
+        const int seg = ch.seg;
         int lpl = 1, lines = 1, rpad = 0;
         size_t l = 0;
-        int rc = pass_geometry(L, F.T[axis], seg, lpl, lines, rpad, l);
+        int rc = pass_geometry(L, F.T[axis], seg, ch.lpf, lpl, lines, rpad, l);
         if (rc) return rc;
-        lds_of[seg] = std::max(lds_of[seg], l);
+        const int gkey = seg * 1024 + ch.lpf;
+        lds_of[gkey] = std::max(lds_of[gkey], l);
         PassField &p = pf[f];
         p.len = L;
         p.T = F.T[axis];
@@ -472,7 +498,7 @@ int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<Pa
             scale_groups((double)(2 * F.T[axis] + 1) + 2 * F.alpha[axis], F.num_iter, F.pscale[axis], p);
         else
             scale_groups(1.0, F.num_iter, 1.0, p);
-        std::vector<PassWork> &items = by_seg[seg];
+        std::vector<PassWork> &items = by_seg[gkey];
         if (axis == 0) {
             p.line_stride = W;
             p.elem_stride = 1;
@@ -496,7 +522,7 @@ int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<Pa
     groups.clear();
     for (auto &kv : by_seg) {
         std::vector<PassWork> items = axis == 0 ? kv.second : xcd_order(kv.second);
-        groups.push_back({kv.first, (int)work.size(), (int)items.size(), lds_of[kv.first]});
+        groups.push_back({kv.first / 1024, kv.first % 1024, (int)work.size(), (int)items.size(), lds_of[kv.first]});
         work.insert(work.end(), items.begin(), items.end());
     }
     if (work.size() > (size_t)INT32_MAX) return fail(FB_ERR_UNSUPPORTED, "too many work items");
@@ -523,7 +549,7 @@ int run_pass(fb_plan *P, int axis, bool inject, const uint32_t *vals, const doub
 {
     for (const PassGroup &g : P->groups[axis]) {
         int rc = launch_fast_pass(inject, g.seg, P->d_fields, P->d_offsets, P->d_work[axis] + g.first, g.count,
-                                  P->d_pass[axis], kFastThreads, g.lds, nullptr, vals, P->d_line_start,
+                                  P->d_pass[axis], 2 * g.lpf, g.lds, nullptr, vals, P->d_line_start,
                                   P->d_line_count, pts, val, P->d_v, P->d_w, out, wsave, final_pass, s);
         if (rc) return rc;
     }
@@ -849,7 +875,7 @@ static int time_pass_subset(fb_plan *P, int a, const std::vector<PassWork> &work
         const int first = (int)fw.size();
         for (int i = g.first; i < g.first + g.count; ++i)
             if (sel[work[i].field]) fw.push_back(work[i]);
-        if ((int)fw.size() > first) fg.push_back({g.seg, first, (int)fw.size() - first, g.lds});
+        if ((int)fw.size() > first) fg.push_back({g.seg, g.lpf, first, (int)fw.size() - first, g.lds});
     }
     *ms = 0.f;
     if (fw.empty()) return FB_OK;
@@ -868,7 +894,7 @@ static int time_pass_subset(fb_plan *P, int a, const std::vector<PassWork> &work
     for (int rep = 0; rep < 4 && !rc; ++rep) {
         if (rep == 1) (void)hipEventRecord(e0, s);
         for (const PassGroup &g : fg) {
-            rc = launch_fast_pass(a == 0, g.seg, P->d_fields, P->d_offsets, dw + g.first, g.count, dp, kFastThreads,
+            rc = launch_fast_pass(a == 0, g.seg, P->d_fields, P->d_offsets, dw + g.first, g.count, dp, 2 * g.lpf,
                                   g.lds, nullptr, P->sorted_vals, P->d_line_start, P->d_line_count, pts, val, P->d_v,
                                   P->d_w, out, nullptr, fin, s);
             if (rc) break;
@@ -901,14 +927,15 @@ int fb_plan_autotune(fb_plan *P, const double *d_pts, const double *d_val, float
             sel[f] = 1;
         }
         for (auto &cl : classes) {
-            int best = -1;
+            PassChoice best{-1, kBlock};
             float best_ms = 0.f;
+            for (int lpf : kLpfs)
             for (int seg : kSegs) {
                 bool ok = true;
                 for (int f = 0; f < P->nfields && ok; ++f)
-                    if (cl.second[f]) pass_cost(cl.first, P->fh[f].T[a], seg, &ok);
+                    if (cl.second[f]) pass_cost(cl.first, P->fh[f].T[a], seg, lpf, &ok);
                 if (!ok) continue;
-                P->seg_choice[{a, cl.first}] = seg;
+                P->seg_choice[{a, cl.first}] = PassChoice{seg, lpf};
                 std::vector<PassWork> work;
                 std::vector<PassField> pf;
                 std::vector<PassGroup> groups;
@@ -916,12 +943,12 @@ int fb_plan_autotune(fb_plan *P, const double *d_pts, const double *d_val, float
                 float ms = 0.f;
                 if ((rc = time_pass_subset(P, a, work, pf, groups, cl.second, d_pts, d_val, d_out, s, &ms)))
                     return rc;
-                if (best < 0 || ms < best_ms) {
-                    best = seg;
+                if (best.seg < 0 || ms < best_ms) {
+                    best = PassChoice{seg, lpf};
                     best_ms = ms;
                 }
             }
-            if (best < 0) P->seg_choice.erase({a, cl.first});
+            if (best.seg < 0) P->seg_choice.erase({a, cl.first});
             else P->seg_choice[{a, cl.first}] = best;
         }
     }
@@ -931,7 +958,8 @@ int fb_plan_autotune(fb_plan *P, const double *d_pts, const double *d_val, float
     return FB_OK;
 }
 
-int fb_plan_pass_groups(fb_plan *P, int axis, int32_t *seg, int32_t *items, int max_groups, int *ngroups)
+int fb_plan_pass_groups(fb_plan *P, int axis, int32_t *seg, int32_t *threads, int32_t *items, int max_groups,
+                        int *ngroups)
 {
     if (!P || !ngroups) return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument");
     if (axis < 0 || axis > 2) return fail(FB_ERR_INVALID_ARGUMENT, "axis must be 0, 1 or 2");
@@ -939,6 +967,7 @@ int fb_plan_pass_groups(fb_plan *P, int axis, int32_t *seg, int32_t *items, int 
     *ngroups = (int)g.size();
     for (int i = 0; i < (int)g.size() && i < max_groups; ++i) {
         if (seg) seg[i] = g[i].seg;
+        if (threads) threads[i] = 2 * g[i].lpf;
         if (items) items[i] = g[i].count;
     }
     return FB_OK;
@@ -1249,7 +1278,7 @@ int fb_accumulate_lines(int tail, int precision, int64_t nlines, int64_t len, in
         int lpl, lines, rpad;
         size_t lds;
         const int seg = model_seg(len, F.T[0]);
-        if ((rc = pass_geometry(len, F.T[0], seg, lpl, lines, rpad, lds))) return rc;
+        if ((rc = pass_geometry(len, F.T[0], seg, kBlock, lpl, lines, rpad, lds))) return rc;
         PassField pf{};
         pf.seg = seg;
         pf.len = len;
@@ -1270,7 +1299,7 @@ int fb_accumulate_lines(int tail, int precision, int64_t nlines, int64_t len, in
         if ((rc = launch_zero(d_off, 8, s))) return rc;
         if ((rc = launch_f64_to_f32(d_io, d_v, cells, s))) return rc;
         if ((rc = launch_zero(d_w, cells * 4, s))) return rc;
-        if ((rc = launch_fast_pass(false, seg, d_f, d_off, d_work, (int)work.size(), d_pf, kFastThreads, lds, nullptr, nullptr,
+        if ((rc = launch_fast_pass(false, seg, d_f, d_off, d_work, (int)work.size(), d_pf, 2 * kBlock, lds, nullptr, nullptr,
                                    nullptr, nullptr, nullptr, nullptr, d_v, d_w, nullptr, nullptr, 0, s)))
             return rc;
         if ((rc = launch_f32_to_f64(d_v, d_io, cells, 1.0, s))) return rc;
@@ -1362,7 +1391,7 @@ int fb_convolve_grid(int tail, int precision, int dim, const int64_t *size, cons
             FB_HIP_CHECK(hipMemcpy(d_work, work.data(), work.size() * sizeof(PassWork), hipMemcpyHostToDevice));
             FB_HIP_CHECK(hipMemcpy(d_pf, pf.data(), sizeof(PassField), hipMemcpyHostToDevice));
             for (const PassGroup &g : groups)
-                if ((rc = launch_fast_pass(false, g.seg, d_f, d_off, d_work + g.first, g.count, d_pf, kFastThreads,
+                if ((rc = launch_fast_pass(false, g.seg, d_f, d_off, d_work + g.first, g.count, d_pf, 2 * g.lpf,
                                            g.lds, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, d_v, d_w,
                                            nullptr, nullptr, 0, s)))
                     return rc;

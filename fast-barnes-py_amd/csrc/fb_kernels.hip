@@ -536,9 +536,9 @@ __device__ __forceinline__ void tile_copy(float *__restrict__ g, int64_t ls, int
     }
 }
 
-// One workgroup = kFastBlock threads: the first half works on the v field, the second half
-// on the w field of the same nlines lines (each half: lines_cap = 256/lpl lines of lpl
-// lanes).  LDS holds the v lines, then the w lines, each padded by T+1 zeros per side.
+// One workgroup = 128, 256 or 512 threads: the first half works on the v field, the second
+// half on the w field of the same nlines lines (each half: up to lines_cap lines of lpl
+// lanes).  LDS holds the v lines, then the w lines, each padded by zeros on both sides.
 template <bool INJECT, int SEG>
 __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     const FieldDev *__restrict__ fields, const double *__restrict__ offsets, const PassWork *__restrict__ work,
@@ -1007,7 +1007,7 @@ int launch_fast_pass(bool inject, int seg, const FieldDev *d_fields, const doubl
                      int final_pass, hipStream_t s)
 {
     if (nwork <= 0) return FB_OK;
-    if (threads != kFastBlock) {
+    if (threads != 128 && threads != 256 && threads != kFastBlock) {
         set_error("fast pass: unsupported workgroup size");
         return FB_ERR_UNSUPPORTED;
     }

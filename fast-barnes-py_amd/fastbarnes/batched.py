@@ -123,12 +123,14 @@ class BarnesBatch:
         check(lib().fb_plan_autotune(self._plan, d_pts, d_val, d_out, stream))
 
     def pass_groups(self, axis):
-        """[(segment length, work items)] of the launches of the pass along `axis`."""
-        seg = (ctypes.c_int32 * 16)()
-        items = (ctypes.c_int32 * 16)()
+        """[(segment length, workgroup threads, work items)] of the launches of the pass
+        along `axis`."""
+        seg = (ctypes.c_int32 * 32)()
+        thr = (ctypes.c_int32 * 32)()
+        items = (ctypes.c_int32 * 32)()
         n = ctypes.c_int()
-        check(lib().fb_plan_pass_groups(self._plan, axis, seg, items, 16, ctypes.byref(n)))
-        return [(seg[i], items[i]) for i in range(min(n.value, 16))]
+        check(lib().fb_plan_pass_groups(self._plan, axis, seg, thr, items, 32, ctypes.byref(n)))
+        return [(seg[i], thr[i], items[i]) for i in range(min(n.value, 32))]
 
     def set_timing(self, enable):
         check(lib().fb_plan_set_timing(self._plan, int(bool(enable))))

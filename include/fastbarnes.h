@@ -173,13 +173,16 @@ int fb_plan_backward(fb_plan *plan, const double *d_pts, const double *d_val, co
 
 /* Autotuner of the FP32C line passes (no reference counterpart: the reference has no
  * GPU tiling).  For every axis and line length of the plan it times each segment length
- * (elements per lane; it sets the lanes per line and the lines per workgroup) on the plan's
+ * (elements per lane; it sets the lanes per line and the lines per workgroup) and each
+ * workgroup size (128, 256 or 512 threads, LDS scaled to keep 16 waves per CU) on the plan's
  * own buffers and keeps the fastest; untuned plans use a cost model.  Leaves d_out
  * undefined: execute again afterwards.  A no-op for FP64 plans. */
 int fb_plan_autotune(fb_plan *plan, const double *d_pts, const double *d_val, float *d_out, void *stream);
 
-/* The launches of the fast pass along `axis`: segment length and work items of each. */
-int fb_plan_pass_groups(fb_plan *plan, int axis, int32_t *seg, int32_t *items, int max_groups, int *ngroups);
+/* The launches of the fast pass along `axis`: segment length, workgroup size (threads) and
+ * work items of each. */
+int fb_plan_pass_groups(fb_plan *plan, int axis, int32_t *seg, int32_t *threads, int32_t *items, int max_groups,
+                        int *ngroups);
 
 /* Per-stage device time (ms) of the last fb_plan_execute run with timing enabled,
  * measured with HIP events recorded on the execution stream.  names receives a

@@ -130,7 +130,8 @@ def test_autotune_keeps_results_and_reports_groups():
         batch.autotune(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr)
         for axis in range(len(dim_group[0]['size'])):
             groups = batch.pass_groups(axis)
-            assert groups and all(seg in (25, 29, 33, 37, 41, 45, 49) and n > 0 for seg, n in groups)
+            assert groups and all(seg in (25, 29, 33, 37, 41, 45, 49) and thr in (128, 256, 512) and n > 0
+                                  for seg, thr, n in groups)
         batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr, graph=True)
         s.synchronize()
         after = d_out.numpy()
