@@ -67,6 +67,64 @@ __global__ __launch_bounds__(kBlock) void k_offsets(const FieldDev *__restrict__
     if (tid == 0) offsets[blockIdx.x] = snan ? __builtin_nan("") : (smin[0] + smax[0]) / 2.0;
 }
 
+// The same over many workgroups per field (large fields): chunk c of field f (blockIdx.x,
+// blockIdx.y) reduces kOffChunk values to (min, max, NaN flag); k_offsets_final combines a
+// field's chunks.  min/max do not depend on the order, so the result equals k_offsets'.
+constexpr int kOffChunk = 8192;
+
+__global__ __launch_bounds__(kBlock) void k_offsets_part(const FieldDev *__restrict__ fields,
+                                                         const double *__restrict__ val, double *__restrict__ part)
+{
+    __shared__ double smin[kBlock], smax[kBlock];
+    __shared__ int snan;
+    const FieldDev &F = fields[blockIdx.y];
+    const int tid = threadIdx.x;
+    if (tid == 0) snan = 0;
+    const int64_t b0 = (int64_t)blockIdx.x * kOffChunk;
+    const int64_t b1 = min(F.n, b0 + kOffChunk);
+    double mn = INFINITY, mx = -INFINITY;
+    int isn = 0;
+    for (int64_t k = b0 + tid; k < b1; k += kBlock) {
+        const double v = val[F.pt_off + k];
+        if (v != v) isn = 1;
+        mn = fmin(mn, v);
+        mx = fmax(mx, v);
+    }
+    smin[tid] = mn;
+    smax[tid] = mx;
+    __syncthreads();
+    if (isn) snan = 1;
+    for (int st = kBlock / 2; st > 0; st >>= 1) {
+        if (tid < st) {
+            smin[tid] = fmin(smin[tid], smin[tid + st]);
+            smax[tid] = fmax(smax[tid], smax[tid + st]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        double *o = part + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 3;
+        o[0] = smin[0];
+        o[1] = smax[0];
+        o[2] = snan ? 1.0 : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_offsets_final(const double *__restrict__ part, int nfields, int nchunk,
+                                                          double *__restrict__ offsets)
+{
+    const int f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= nfields) return;
+    double mn = INFINITY, mx = -INFINITY;
+    bool isn = false;
+    for (int c = 0; c < nchunk; ++c) {
+        const double *o = part + ((int64_t)f * nchunk + c) * 3;
+        mn = fmin(mn, o[0]);
+        mx = fmax(mx, o[1]);
+        isn = isn || o[2] != 0.0;
+    }
+    offsets[f] = isn ? __builtin_nan("") : (mn + mx) / 2.0;
+}
+
 // ---------------------------------------------------------------------------
 // grid position of a point: xc = (p - x0)/step per axis, dropped when outside
 // [0, size-1) on any axis (interpolation.py:247-250).  NaN coordinates are
@@ -901,9 +959,23 @@ inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock
 // ---------------------------------------------------------------------------
 // launchers
 
-int launch_offsets(const FieldDev *d_fields, int nfields, const double *d_val, double *d_offsets, hipStream_t s)
+int64_t offsets_part_elems(int nfields, int64_t max_n)
 {
-    hipLaunchKernelGGL(k_offsets, dim3(nfields), dim3(kBlock), 0, s, d_fields, d_val, d_offsets);
+    return (int64_t)nfields * ((max_n + kOffChunk - 1) / kOffChunk) * 3;
+}
+
+int launch_offsets(const FieldDev *d_fields, int nfields, int64_t max_n, const double *d_val, double *d_part,
+                   double *d_offsets, hipStream_t s)
+{
+    if (!d_part || max_n <= kOffChunk) {
+        hipLaunchKernelGGL(k_offsets, dim3(nfields), dim3(kBlock), 0, s, d_fields, d_val, d_offsets);
+    } else {
+        const int nchunk = (int)((max_n + kOffChunk - 1) / kOffChunk);
+        hipLaunchKernelGGL(k_offsets_part, dim3(nchunk, nfields), dim3(kBlock), 0, s, d_fields, d_val, d_part);
+        FB_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_offsets_final, dim3((nfields + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                           (const double *)d_part, nfields, nchunk, d_offsets);
+    }
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }

@@ -15,8 +15,11 @@ int radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_
                      uint32_t **vals_out);
 
 // fb_kernels.hip -- injection front end
-int launch_offsets(const FieldDev *d_fields, int nfields, const double *d_val, double *d_offsets,
-                   hipStream_t stream);
+// (min+max)/2 per field; with d_part (offsets_part_elems doubles) large fields are reduced
+// by many workgroups
+int64_t offsets_part_elems(int nfields, int64_t max_n);
+int launch_offsets(const FieldDev *d_fields, int nfields, int64_t max_n, const double *d_val, double *d_part,
+                   double *d_offsets, hipStream_t stream);
 // one (base cell, point) pair per point; per-base-row point counts (row = x-line index)
 int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
                       const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_row_count,

@@ -13,7 +13,9 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kItems = 16;
-constexpr int kTile = kThreads * kItems;   // 4096 elements per workgroup
+constexpr int kTile = kThreads * kItems;   // 4096 elements per workgroup of the multi-level scan
+constexpr int kRItems = 4;
+constexpr int kRTile = kThreads * kRItems; // 1024 elements per workgroup of the radix passes
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kWaves = kThreads / 64;
@@ -69,8 +71,8 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const uint32_t *__restr
     const int tid = threadIdx.x;
     h[tid] = 0u;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kTile;
-    for (int r = 0; r < kItems; ++r) {
+    const int64_t base = (int64_t)blockIdx.x * kRTile;
+    for (int r = 0; r < kRItems; ++r) {
         const int64_t idx = base + (int64_t)r * kThreads + tid;
         if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (kRadix - 1)], 1u);
     }
@@ -91,8 +93,8 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const uint32_t *__re
     const int wave = tid >> 6;
     const uint64_t lanes_below = lane ? (~0ull >> (64 - lane)) : 0ull;
     running[tid] = offs[(int64_t)tid * ntiles + blockIdx.x];
-    const int64_t base = (int64_t)blockIdx.x * kTile;
-    for (int r = 0; r < kItems; ++r) {
+    const int64_t base = (int64_t)blockIdx.x * kRTile;
+    for (int r = 0; r < kRItems; ++r) {
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0u;
         __syncthreads();
@@ -162,7 +164,7 @@ int scan_exclusive(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tmp, 
     return FB_OK;
 }
 
-int64_t radix_hist_elems(int64_t n) { return (int64_t)kRadix * cdiv(n > 0 ? n : 1, kTile); }
+int64_t radix_hist_elems(int64_t n) { return (int64_t)kRadix * cdiv(n > 0 ? n : 1, kRTile); }
 
 int radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, int64_t n, int bits,
                      uint32_t *hist, uint32_t *scan_tmp, hipStream_t stream, uint32_t **keys_out,
@@ -170,7 +172,7 @@ int radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_
 {
     uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
     if (n > 0) {
-        const int ntiles = (int)cdiv(n, kTile);
+        const int ntiles = (int)cdiv(n, kRTile);
         const int64_t nh = (int64_t)kRadix * ntiles;
         for (int shift = 0; shift < bits; shift += kRadixBits) {
             hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kThreads), 0, stream, (const uint32_t *)ki, n,
