@@ -205,8 +205,9 @@ def leg_train(cfgs, steps, warmup):
 
 def leg_real(steps):
     """The paper's RMSE-versus-iterations curve (figure 8) on the 3490 real QFF stations at
-    2400x1200: num_iter 1..50 for both convolution methods, each method one FP32C batch of 50
-    fields, against naive Barnes (GPU, FP64); the FP64 path's curve beside it."""
+    2400x1200: num_iter 1..50 for both convolution methods, each method one FP32C batch, against
+    naive Barnes (GPU, FP64); the FP64 path's curve beside it.  The plain-convolution counts
+    whose integer threshold wraps (FP32C refuses them) run in an FP64 batch."""
     from fastbarnes import hip, realdata
     log('real: figure-8 curve, 3490 stations, 2400x1200')
     pts, val = realdata.read_csv_array(os.path.join(ROOT, 'tests', 'golden', 'qff', 'PressQFF_202007271200_3490.csv'))
@@ -217,12 +218,14 @@ def leg_real(steps):
     t_curve = time.perf_counter() - t0
     c64, _ = realdata.rmse_curve(pts, val, 32.0, num_iters=ns, precision='fp64', naive=naive)
     pick = [1, 2, 3, 4, 5, 10, 20, 50]
+    wr = realdata.fp64_counts('convolution', 32.0, num_iters=ns)
     diff = max(abs(a - b) for m in c32 for a, b in zip(c32[m], c64[m]))
     return dict(workload='3490 real QFF stations, paper grid 2400x1200 (x0=(-26+1/32, 34.5)), sigma=1; '
                          'RMSE over [-7,5]x[36,56] against naive Barnes',
                 seconds_whole_curve_fp32c_incl_naive_and_host_io=round(t_curve, 3),
                 rmse_conv={n: round(c32['convolution'][n - 1], 5) for n in pick},
                 rmse_opt_conv={n: round(c32['optimized_convolution'][n - 1], 5) for n in pick},
+                convolution_counts_in_fp64='%d..%d' % (wr[0], wr[-1]) if wr else 'none',
                 max_abs_rmse_diff_fp32c_vs_fp64=diff)
 
 

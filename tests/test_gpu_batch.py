@@ -141,3 +141,21 @@ def test_autotune_keeps_results_and_reports_groups():
         batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr, graph=False)
         s.synchronize()
         _bitwise(d_out.numpy(), after)
+
+
+def test_full_size_batch_matches_single_fields():
+    """Full-size fields (2400x1200 in 2D, 256x256x64 in 3D) in one batch: every stage runs once
+    over all fields with a work list far larger than the GPU, and every field equals its
+    single-field run bit for bit."""
+    from fastbarnes import workloads
+    from fastbarnes.batched import BarnesBatch
+    from fastbarnes.interpolation import barnes
+    fs = [workloads.config2(seed=90 + i) for i in range(3)]
+    fs += [workloads.config4(seed=95 + i, n=20000, size=(256, 256, 64)) for i in range(4)]
+    for group in (fs[:3], fs[3:]):
+        batch = BarnesBatch([_desc(c) for c in group], precision='fp32c')
+        outs = batch.interpolate([c['pts'] for c in group], [c['val'] for c in group])
+        for c, o in zip(group, outs):
+            ref = barnes(c['pts'], c['val'], c['sigma'], c['x0'], c['step'], c['size'], num_iter=c['num_iter'],
+                         max_dist=c['max_dist'], precision='fp32c')
+            _bitwise(o, ref)

@@ -66,6 +66,24 @@ def test_real_rmse_curve_matches_oracle(orc):
     assert all(a > b for a, b in zip(oc, oc[1:]))
 
 
+def test_real_rmse_curve_fp32c_routes_wrapped_counts_to_fp64():
+    """With FP32C, the curve's plain-convolution counts whose integer threshold wraps (K**n >=
+    2**63: n >= 23 at resolution 8) run in an FP64 batch and equal the FP64 curve there exactly;
+    the other counts stay within FP32C rounding of the FP64 curve."""
+    pts, val = realdata.read_csv_array(QFF)
+    ns = [1, 4, 10, 22, 23, 50]
+    wrapped = realdata.fp64_counts('convolution', 8.0, num_iters=ns)
+    assert wrapped == [23, 50] and realdata.fp64_counts('optimized_convolution', 8.0, num_iters=ns) == []
+    c32, naive = realdata.rmse_curve(pts, val, resolution=8.0, num_iters=ns, precision='fp32c')
+    c64, _ = realdata.rmse_curve(pts, val, resolution=8.0, num_iters=ns, precision='fp64', naive=naive)
+    for method in ('convolution', 'optimized_convolution'):
+        for k, a, b in zip(ns, c32[method], c64[method]):
+            if method == 'convolution' and k in wrapped:
+                assert a == b
+            else:
+                assert abs(a - b) <= 1e-5, (method, k, a, b)
+
+
 @pytest.mark.parametrize('method,num_iter', [('optimized_convolution', 50), ('optimized_convolution', 30),
                                              ('convolution', 12)])
 def test_real_fp32c_many_iterations(method, num_iter):
