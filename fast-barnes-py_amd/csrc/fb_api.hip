@@ -285,23 +285,79 @@ int model_seg(int64_t L, int T, int lpf = kBlock)
     return best;
 }
 
-// segment length and lanes per field of a pass (autotuned or from the cost model)
+// Segment lengths of the prefix-pass instantiations (FB_PFX_SEGS of fb_kernels.hip).
+constexpr int kPfxSegs[] = {15, 17, 19, 21, 23};
+
+// Geometry of a prefix pass: lpl lanes per line, `lines` lines per field in a workgroup of
+// 2*lpf threads (all lanes busy unless the LDS limit cuts lines), LDS bytes, and the
+// workgroups per CU the LDS allows.  A line takes 8 bytes per position (FP64 prefix sums),
+// so the LDS, not the thread count, bounds how many lines a CU holds.
+int prefix_geometry(int64_t L, int T, int seg, int lpf, int &lpl, int &lines, size_t &lds, int &wg_per_cu)
+{
+    const int64_t need = (L + seg - 1) / seg;
+    lpl = 1;
+    while (lpl < need) lpl <<= 1;
+    if (lpl > lpf)
+        return fail(FB_ERR_UNSUPPORTED, "line length " + std::to_string(L) + " exceeds the prefix-pass limit of " +
+                                            std::to_string((int64_t)lpf * seg) + " for this workgroup size");
+    const size_t per_line = (size_t)prefix_line_stride(lpl, seg, T) * 2 * sizeof(double);   // v and w
+    const size_t reserve = 16 * sizeof(double);   // wave totals of the segment scan
+    int n = lpf / lpl;
+    while (n > 1 && n * per_line + reserve > kLdsMax) --n;
+    if (n >= 16) n &= ~3;   // float4 column tiles
+    lines = n;
+    lds = round_bytes(n * per_line + reserve);
+    if (lds > kLdsMax)
+        return fail(FB_ERR_UNSUPPORTED, "line of length " + std::to_string(L) + " with half kernel " +
+                                            std::to_string(T) + " does not fit the local data share");
+    const int waves = 2 * lpf / 64;
+    wg_per_cu = (int)std::min<size_t>(kLdsMax / lds, (size_t)(32 / std::max(waves, 1)));
+    return FB_OK;
+}
+
+// Cost model of a prefix pass per line, in the units of pass_cost: about 5.5 instructions
+// per element (two differences, two FMAs of the chain, the store of base + chain) plus ~60
+// per iteration (segment-total scan, barriers, addressing), divided by the lines a
+// workgroup holds; below 12 waves per CU the FP64 latency is no longer hidden.
+double prefix_cost(int64_t L, int T, int seg, int lpf = kBlock, bool *ok = nullptr)
+{
+    int lpl, lines, wgs;
+    size_t lds;
+    const std::string keep = last_error();
+    const int rc = prefix_geometry(L, T, seg, lpf, lpl, lines, lds, wgs);
+    set_error(keep);
+    if (ok) *ok = rc == FB_OK;
+    if (rc) return 1e300;
+    const double waves = (double)wgs * (2 * lpf / 64);
+    const double occ = std::min(1.0, waves / 12.0);
+    return (5.5 * seg + 60.0) / lines * (kBlock / lpf) / occ;
+}
+
+// segment length, lanes per field and kind of a pass (autotuned or from the cost model)
 struct PassChoice {
     int seg;
     int lpf;
+    int kind;
 };
 
-// Untuned default: 256-thread workgroups (faster than 512 on every measured workload) when
-// a segment length fits the line in 128 lanes, else 512-thread workgroups.
+double choice_cost(int64_t L, int T, const PassChoice &c, bool *ok = nullptr)
+{
+    return c.kind == kPassPrefix ? prefix_cost(L, T, c.seg, c.lpf, ok) : pass_cost(L, T, c.seg, c.lpf, ok);
+}
+
+// Untuned default: the sliding-window pass with 256-thread workgroups (faster than 512 on
+// every measured workload) when a segment length fits the line in 128 lanes, else
+// 512-thread workgroups.  The prefix pass is an autotuner candidate only: it halves the
+// elements a CU holds in LDS (FP64) and was slower on the 2400x1200 and 512^2x128 fields.
 PassChoice model_choice(int64_t L, int T)
 {
     for (int lpf : {kBlock / 2, kBlock}) {
         const int seg = model_seg(L, T, lpf);
         bool ok = false;
         pass_cost(L, T, seg, lpf, &ok);
-        if (ok) return PassChoice{seg, lpf};
+        if (ok) return PassChoice{seg, lpf, kPassSlide};
     }
-    return PassChoice{41, kBlock};   // pass_geometry reports the limit
+    return PassChoice{41, kBlock, kPassSlide};   // pass_geometry reports the limit
 }
 
 // order work items so that blockIdx b runs tile t with consecutive tiles on one XCD
@@ -339,6 +395,7 @@ namespace {
 struct PassGroup {
     int seg;
     int lpf;      // lanes per field: the workgroup has 2*lpf threads
+    int kind;     // kPassSlide or kPassPrefix
     int first;
     int count;
     size_t lds;
@@ -466,8 +523,9 @@ int make_field(const fb_field &f, FieldDev &d, int fast)
 }
 
 // Work list and per-field geometry of the fast pass along `axis`: each field runs with the
-// segment length chosen for its line length (autotuned choice, else the cost model); the
-// work items are grouped by segment length, one launch per group.
+// kind and segment length chosen for its line length (autotuned choice, else the cost
+// model); the work items are grouped by (kind, segment length, workgroup size), one launch
+// per group.
 int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<PassField> &pf,
                std::vector<PassGroup> &groups)
 {
@@ -481,11 +539,12 @@ int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<Pa
         auto it = P->seg_choice.find({axis, L});
         const PassChoice ch = it != P->seg_choice.end() ? it->second : model_choice(L, F.T[axis]);
         const int seg = ch.seg;
-        int lpl = 1, lines = 1, rpad = 0;
+        int lpl = 1, lines = 1, rpad = 0, wgs = 0;
         size_t l = 0;
-        int rc = pass_geometry(L, F.T[axis], seg, ch.lpf, lpl, lines, rpad, l);
+        int rc = ch.kind == kPassPrefix ? prefix_geometry(L, F.T[axis], seg, ch.lpf, lpl, lines, l, wgs)
+                                        : pass_geometry(L, F.T[axis], seg, ch.lpf, lpl, lines, rpad, l);
         if (rc) return rc;
-        const int gkey = seg * 1024 + ch.lpf;
+        const int gkey = (ch.kind * 1024 + seg) * 1024 + ch.lpf;
         lds_of[gkey] = std::max(lds_of[gkey], l);
         PassField &p = pf[f];
         p.len = L;
@@ -523,7 +582,8 @@ int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<Pa
     groups.clear();
     for (auto &kv : by_seg) {
         std::vector<PassWork> items = axis == 0 ? kv.second : xcd_order(kv.second);
-        groups.push_back({kv.first / 1024, kv.first % 1024, (int)work.size(), (int)items.size(), lds_of[kv.first]});
+        groups.push_back({(kv.first / 1024) % 1024, kv.first % 1024, kv.first / (1024 * 1024), (int)work.size(),
+                          (int)items.size(), lds_of[kv.first]});
         work.insert(work.end(), items.begin(), items.end());
     }
     if (work.size() > (size_t)INT32_MAX) return fail(FB_ERR_UNSUPPORTED, "too many work items");
@@ -549,7 +609,7 @@ int run_pass(fb_plan *P, int axis, bool inject, const uint32_t *vals, const doub
              float *out, float *wsave, int final_pass, hipStream_t s)
 {
     for (const PassGroup &g : P->groups[axis]) {
-        int rc = launch_fast_pass(inject, g.seg, P->d_fields, P->d_offsets, P->d_work[axis] + g.first, g.count,
+        int rc = launch_fast_pass(g.kind, inject, g.seg, P->d_fields, P->d_offsets, P->d_work[axis] + g.first, g.count,
                                   P->d_pass[axis], 2 * g.lpf, g.lds, nullptr, vals, P->d_line_start,
                                   P->d_line_count, pts, val, P->d_v, P->d_w, out, wsave, final_pass, s);
         if (rc) return rc;
@@ -878,7 +938,7 @@ static int time_pass_subset(fb_plan *P, int a, const std::vector<PassWork> &work
         const int first = (int)fw.size();
         for (int i = g.first; i < g.first + g.count; ++i)
             if (sel[work[i].field]) fw.push_back(work[i]);
-        if ((int)fw.size() > first) fg.push_back({g.seg, g.lpf, first, (int)fw.size() - first, g.lds});
+        if ((int)fw.size() > first) fg.push_back({g.seg, g.lpf, g.kind, first, (int)fw.size() - first, g.lds});
     }
     *ms = 0.f;
     if (fw.empty()) return FB_OK;
@@ -897,7 +957,7 @@ static int time_pass_subset(fb_plan *P, int a, const std::vector<PassWork> &work
     for (int rep = 0; rep < 6 && !rc; ++rep) {
         if (rep == 1) (void)hipEventRecord(e0, s);
         for (const PassGroup &g : fg) {
-            rc = launch_fast_pass(a == 0, g.seg, P->d_fields, P->d_offsets, dw + g.first, g.count, dp, 2 * g.lpf,
+            rc = launch_fast_pass(g.kind, a == 0, g.seg, P->d_fields, P->d_offsets, dw + g.first, g.count, dp, 2 * g.lpf,
                                   g.lds, nullptr, P->sorted_vals, P->d_line_start, P->d_line_count, pts, val, P->d_v,
                                   P->d_w, out, nullptr, fin, s);
             if (rc) break;
@@ -935,19 +995,24 @@ int fb_plan_autotune(fb_plan *P, const double *d_pts, const double *d_val, float
             int f0 = 0;
             while (!cl.second[f0]) ++f0;
             const PassChoice base = model_choice(cl.first, P->fh[f0].T[a]);
-            PassChoice best{-1, kBlock};
+            PassChoice best{-1, kBlock, kPassSlide};
             float best_ms = 0.f;
             std::vector<PassChoice> cands{base};
-            for (int lpf : kLpfs)
+            auto same = [](const PassChoice &x, const PassChoice &y) {
+                return x.seg == y.seg && x.lpf == y.lpf && x.kind == y.kind;
+            };
+            for (int lpf : kLpfs) {
+                for (int seg : kPfxSegs)
+                    if (!same({seg, lpf, kPassPrefix}, base)) cands.push_back({seg, lpf, kPassPrefix});
                 for (int seg : kSegs)
-                    if (seg != base.seg || lpf != base.lpf) cands.push_back({seg, lpf});
+                    if (!same({seg, lpf, kPassSlide}, base)) cands.push_back({seg, lpf, kPassSlide});
+            }
             for (const PassChoice &cand : cands) {
-                const int seg = cand.seg, lpf = cand.lpf;
                 bool ok = true;
                 for (int f = 0; f < P->nfields && ok; ++f)
-                    if (cl.second[f]) pass_cost(cl.first, P->fh[f].T[a], seg, lpf, &ok);
+                    if (cl.second[f]) choice_cost(cl.first, P->fh[f].T[a], cand, &ok);
                 if (!ok) continue;
-                P->seg_choice[{a, cl.first}] = PassChoice{seg, lpf};
+                P->seg_choice[{a, cl.first}] = cand;
                 std::vector<PassWork> work;
                 std::vector<PassField> pf;
                 std::vector<PassGroup> groups;
@@ -956,7 +1021,7 @@ int fb_plan_autotune(fb_plan *P, const double *d_pts, const double *d_val, float
                 if ((rc = time_pass_subset(P, a, work, pf, groups, cl.second, d_pts, d_val, d_out, s, &ms)))
                     return rc;
                 if (best.seg < 0 || ms < 0.98f * best_ms) {
-                    best = PassChoice{seg, lpf};
+                    best = cand;
                     best_ms = ms;
                 }
             }
@@ -970,8 +1035,46 @@ int fb_plan_autotune(fb_plan *P, const double *d_pts, const double *d_val, float
     return FB_OK;
 }
 
-int fb_plan_pass_groups(fb_plan *P, int axis, int32_t *seg, int32_t *threads, int32_t *items, int max_groups,
-                        int *ngroups)
+int fb_plan_choose_pass(fb_plan *P, int axis, int64_t len, int kind, int seg, int threads)
+{
+    if (!P) return fail(FB_ERR_INVALID_ARGUMENT, "plan is NULL");
+    if (P->precision != FB_PREC_FP32C) return fail(FB_ERR_INVALID_ARGUMENT, "pass choices apply to FP32C plans");
+    if (axis < 0 || axis >= P->dim) return fail(FB_ERR_INVALID_ARGUMENT, "axis out of range");
+    if (kind != kPassSlide && kind != kPassPrefix) return fail(FB_ERR_INVALID_ARGUMENT, "unknown pass kind");
+    if (threads != 128 && threads != 256 && threads != 2 * kBlock)
+        return fail(FB_ERR_INVALID_ARGUMENT, "threads must be 128, 256 or 512");
+    if (!fast_pass_seg_supported(kind, seg))
+        return fail(FB_ERR_INVALID_ARGUMENT, "no pass instantiation for segment length " + std::to_string(seg));
+    const PassChoice ch{seg, threads / 2, kind};
+    bool any = false;
+    for (int f = 0; f < P->nfields; ++f) {
+        if (P->fh[f].size[axis] != len) continue;
+        any = true;
+        bool ok = false;
+        choice_cost(len, P->fh[f].T[axis], ch, &ok);
+        if (!ok) return fail(FB_ERR_UNSUPPORTED, "the pass choice does not fit lines of length " + std::to_string(len));
+    }
+    if (!any) return fail(FB_ERR_INVALID_ARGUMENT, "no field has lines of length " + std::to_string(len));
+    const auto key = std::make_pair(axis, len);
+    const bool had = P->seg_choice.count(key) != 0;
+    const PassChoice old = had ? P->seg_choice[key] : ch;
+    P->seg_choice[key] = ch;
+    int rc = build_passes(P);
+    if (rc) {
+        if (had) P->seg_choice[key] = old;
+        else P->seg_choice.erase(key);
+        const std::string keep = last_error();
+        (void)build_passes(P);
+        set_error(keep);
+        return rc;
+    }
+    if (P->gexec) { (void)hipGraphExecDestroy(P->gexec); P->gexec = nullptr; }
+    if (P->graph) { (void)hipGraphDestroy(P->graph); P->graph = nullptr; }
+    return FB_OK;
+}
+
+int fb_plan_pass_groups(fb_plan *P, int axis, int32_t *seg, int32_t *threads, int32_t *items, int32_t *kind,
+                        int max_groups, int *ngroups)
 {
     if (!P || !ngroups) return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument");
     if (axis < 0 || axis > 2) return fail(FB_ERR_INVALID_ARGUMENT, "axis must be 0, 1 or 2");
@@ -981,6 +1084,7 @@ int fb_plan_pass_groups(fb_plan *P, int axis, int32_t *seg, int32_t *threads, in
         if (seg) seg[i] = g[i].seg;
         if (threads) threads[i] = 2 * g[i].lpf;
         if (items) items[i] = g[i].count;
+        if (kind) kind[i] = g[i].kind;
     }
     return FB_OK;
 }
@@ -1315,7 +1419,8 @@ int fb_accumulate_lines(int tail, int precision, int64_t nlines, int64_t len, in
         if ((rc = launch_zero(d_off, 8, s))) return rc;
         if ((rc = launch_f64_to_f32(d_io, d_v, cells, s))) return rc;
         if ((rc = launch_zero(d_w, cells * 4, s))) return rc;
-        if ((rc = launch_fast_pass(false, seg, d_f, d_off, d_work, (int)work.size(), d_pf, 2 * kBlock, lds, nullptr, nullptr,
+        if ((rc = launch_fast_pass(kPassSlide, false, seg, d_f, d_off, d_work, (int)work.size(), d_pf, 2 * kBlock, lds,
+                                   nullptr, nullptr,
                                    nullptr, nullptr, nullptr, nullptr, d_v, d_w, nullptr, nullptr, 0, s)))
             return rc;
         if ((rc = launch_f32_to_f64(d_v, d_io, cells, 1.0, s))) return rc;
@@ -1407,7 +1512,7 @@ int fb_convolve_grid(int tail, int precision, int dim, const int64_t *size, cons
             FB_HIP_CHECK(hipMemcpy(d_work, work.data(), work.size() * sizeof(PassWork), hipMemcpyHostToDevice));
             FB_HIP_CHECK(hipMemcpy(d_pf, pf.data(), sizeof(PassField), hipMemcpyHostToDevice));
             for (const PassGroup &g : groups)
-                if ((rc = launch_fast_pass(false, g.seg, d_f, d_off, d_work + g.first, g.count, d_pf, 2 * g.lpf,
+                if ((rc = launch_fast_pass(g.kind, false, g.seg, d_f, d_off, d_work + g.first, g.count, d_pf, 2 * g.lpf,
                                            g.lds, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, d_v, d_w,
                                            nullptr, nullptr, 0, s)))
                     return rc;

@@ -45,6 +45,17 @@ struct PassWork {
     int64_t base;
 };
 
+// Doubles per line in the LDS of the prefix pass: positions -T-1 .. lpl*seg+T+1, rounded up
+// to an even count so that every line's float32 view starts 16-byte aligned.
+__host__ __device__ inline int prefix_line_stride(int lpl, int seg, int T)
+{
+    return (lpl * seg + 2 * T + 3 + 1) & ~1;
+}
+
+// Kinds of FP32C line pass: the sliding-window pass (float32 tile, FP64 window sums) and the
+// prefix pass (FP64 prefix sums in LDS).
+enum PassKind { kPassSlide = 0, kPassPrefix = 1 };
+
 // Per-field geometry of one fast line pass along one axis.
 struct PassField {
     int64_t len;           // line length L

@@ -438,24 +438,41 @@ __device__ __forceinline__ void slide1(const float *__restrict__ lin, int s, int
 #pragma unroll
     for (int j = 0; j < kSegArr; ++j) sb[j] = subp[j];
     const float p0 = lin[s - T - 1];
-    // window sum: the first min(K, kSegArr) terms are sb[], the rest (K > kSegArr) from LDS
+    // window sum: the first min(K, kSegArr) terms are sb[], the rest (K > kSegArr) from LDS,
+    // four at a time without per-term conditions (K is the same for the whole workgroup)
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     if (K >= kSegArr) {
-        for (int i = kSegArr; i < K; i += 4) {
+        int i = kSegArr;
+        for (; i + 4 <= K; i += 4) {
             float x[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) x[u] = (i + u < K) ? subp[i + u] : 0.f;
+            for (int u = 0; u < 4; ++u) x[u] = subp[i + u];
 #pragma unroll
             for (int u = 0; u < 4; ++u) v[u] += (double)x[u];
         }
+        for (; i < K; ++i) v[0] += (double)subp[i];
 #pragma unroll
         for (int j = 0; j < kSegArr; ++j) v[j & 3] += (double)sb[j];
     } else {
         // short kernels (2T+1 < kSegArr): direct sum from LDS
-        for (int i = 0; i < K; ++i) v[i & 3] += (double)subp[i];
+        int i = 0;
+        for (; i + 4 <= K; i += 4) {
+            float x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = subp[i + u];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] += (double)x[u];
+        }
+        for (; i < K; ++i) v[0] += (double)subp[i];
     }
     double a = (v[0] + v[1]) + (v[2] + v[3]);
+    // Each step: output = a + alpha*(left + right tail), then a slides by the entering minus
+    // the leaving term, both exact in FP64.  The scaling (SCALE) iteration forms the output in
+    // FP64; the others round a to float32 and add the tails in float32 (one float32 FMA): the
+    // output is stored as float32 either way, and a float32 FMA is half the cost of an FP64 one.
     double p = (double)p0;
+    float pf = p0;
+    const float alpha_f = (float)alpha;
     const float *addp = lin + s + T + 1;
     float cur[kBlk], nxt[kBlk];
 #pragma unroll
@@ -471,15 +488,20 @@ __device__ __forceinline__ void slide1(const float *__restrict__ lin, int s, int
         for (int u = 0; u < kBlk; ++u) {
             const int j = b * kBlk + u;
             if (j < kSegArr) {
+                const float svf = sb[j];
                 const double ad = (double)cur[u];
-                const double sv = (double)sb[j];
-                double o = fma(alpha, p + ad, a);
+                const double sv = (double)svf;
+                float r;
+                if (SCALE) {
+                    r = (float)(fma(alpha, p + ad, a) * scale);
+                    p = sv;
+                } else {
+                    r = fmaf(alpha_f, pf + cur[u], (float)a);
+                    pf = svf;
+                }
                 a += ad - sv;
-                p = sv;
-                if (SCALE) o *= scale;
                 // the empty asm pins each float32 result at its step; otherwise the compiler
                 // sinks all conversions to the store and keeps kSegArr doubles live
-                float r = (float)o;
                 asm volatile("" : "+v"(r));
                 sb[j] = r;
             }
@@ -594,6 +616,72 @@ __device__ __forceinline__ void tile_copy(float *__restrict__ g, int64_t ls, int
     }
 }
 
+// Deterministic fused injection of one line's points into its float32 row (element x at
+// row[x]).  The line's sub-streams (sub_stream: one base row, one corner column),
+// concatenated, are taken in chunks of up to 64 entries by the line's first `iw` lanes
+// (`injector`).  A segmented suffix sum over adjacent lanes of equal target cell (fixed
+// tree order) gives each run's total to its first lane.  Equal cells adjacent in one
+// sub-stream form one run; a cell can head two runs of a chunk only from different
+// sub-streams, so the heads add their totals in one round per sub-stream of the head
+// (program order), without conflicts.  The result does not depend on timing.  Called by
+// every thread of the workgroup (wave-wide shuffles).
+__device__ __forceinline__ void inject_line(const FieldDev &F, int64_t line_in_field, bool injector, int iw, int sid,
+                                            int fsel, double off, const uint32_t *__restrict__ vals,
+                                            const uint32_t *__restrict__ line_start,
+                                            const uint32_t *__restrict__ line_count, const double *__restrict__ pts,
+                                            const double *__restrict__ val, float *row)
+{
+    const int nsub = 2 * source_rows(F.dim);
+    uint32_t rsq[8], pre[9];
+    pre[0] = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        uint32_t rs = 0, rc = 0;
+        if (injector && q < nsub) sub_stream(F, line_in_field, q, line_start, line_count, rs, rc);
+        rsq[q] = rs;
+        pre[q + 1] = pre[q] + rc;
+    }
+    int nch = (int)((pre[8] + iw - 1) / iw);
+    for (int d = 32; d >= 1; d >>= 1) nch = max(nch, __shfl_xor(nch, d));
+    for (int ch = 0; ch < nch; ++ch) {
+        const uint32_t e = (uint32_t)ch * iw + sid;
+        const bool valid = injector && e < pre[8];
+        int x = -1, hq = 0;
+        double c = 0.0;
+        if (valid) {
+            uint32_t src = 0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (e >= pre[q] && e < pre[q + 1]) {
+                    hq = q;
+                    src = rsq[q] + (e - pre[q]);
+                }
+            const uint32_t p = vals[src];
+            PointPos pp;
+            point_pos(F, pts + (int64_t)p * F.dim, pp);
+            x = (int)pp.i[0] + (hq & 1);
+            const double w = corner_weight(F.dim, pp, hq & 1, (hq >> 1) & 1, hq >> 2);
+            c = fsel ? w : w * (val[p] - off);
+        }
+        // runs: maximal groups of adjacent lanes with equal cell; run id = inclusive count
+        // of run starts (the concatenated stream is not sorted, so equal cells further
+        // apart belong to different runs)
+        const int px = __shfl_up(x, 1, iw);
+        const bool start = sid == 0 || px != x;
+        const int lane = threadIdx.x & 63;
+        const int rid = __popcll(__ballot(start) & ((2ull << lane) - 1ull));
+        for (int d = 1; d < iw; d <<= 1) {
+            const double oc = __shfl_down(c, d, iw);
+            const int orid = __shfl_down(rid, d, iw);
+            if (sid + d < iw && orid == rid) c += oc;
+        }
+        const bool head = valid && start;
+        float *cell = row + (x < 0 ? 0 : x);
+        for (int q = 0; q < nsub; ++q)
+            if (head && hq == q) *cell = (float)((double)*cell + c);
+    }
+}
+
 // One workgroup = 128, 256 or 512 threads: the first half works on the v field, the second
 // half on the w field of the same nlines lines (each half: up to lines_cap lines of lpl
 // lanes).  LDS holds the v lines, then the w lines, each padded by zeros on both sides.
@@ -656,62 +744,8 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     __syncthreads();
 
     if (INJECT) {
-        // Deterministic injection.  The line's sub-streams (sub_stream: one base row, one
-        // corner column), concatenated, are taken in chunks of up to 64 entries by the line's
-        // first lanes.  A segmented suffix sum over adjacent lanes of equal target cell (fixed
-        // tree order) gives each run's total to its first lane.  Equal cells adjacent in one
-        // sub-stream form one run; a cell can head two runs of a chunk only from different
-        // sub-streams, so the heads add their totals to LDS in one round per sub-stream of
-        // the head (program order), without conflicts.  The result does not depend on timing.
-        const int nsub = 2 * source_rows(F.dim);
-        uint32_t rsq[8], pre[9];
-        pre[0] = 0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            uint32_t rs = 0, rc = 0;
-            if (injector && q < nsub) sub_stream(F, line_in_field, q, line_start, line_count, rs, rc);
-            rsq[q] = rs;
-            pre[q + 1] = pre[q] + rc;
-        }
-        int nch = (int)((pre[8] + iw - 1) / iw);
-        for (int d = 32; d >= 1; d >>= 1) nch = max(nch, __shfl_xor(nch, d));
-        for (int ch = 0; ch < nch; ++ch) {
-            const uint32_t e = (uint32_t)ch * iw + sid;
-            const bool valid = injector && e < pre[8];
-            int x = -1, hq = 0;
-            double c = 0.0;
-            if (valid) {
-                uint32_t src = 0;
-#pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    if (e >= pre[q] && e < pre[q + 1]) {
-                        hq = q;
-                        src = rsq[q] + (e - pre[q]);
-                    }
-                const uint32_t p = vals[src];
-                PointPos pp;
-                point_pos(F, pts + (int64_t)p * F.dim, pp);
-                x = (int)pp.i[0] + (hq & 1);
-                const double w = corner_weight(F.dim, pp, hq & 1, (hq >> 1) & 1, hq >> 2);
-                c = fsel ? w : w * (val[p] - off);
-            }
-            // runs: maximal groups of adjacent lanes with equal cell; run id = inclusive count
-            // of run starts (the concatenated stream is not sorted, so equal cells further
-            // apart belong to different runs)
-            const int px = __shfl_up(x, 1, iw);
-            const bool start = sid == 0 || px != x;
-            const int lane = threadIdx.x & 63;
-            const int rid = __popcll(__ballot(start) & ((2ull << lane) - 1ull));
-            for (int d = 1; d < iw; d <<= 1) {
-                const double oc = __shfl_down(c, d, iw);
-                const int orid = __shfl_down(rid, d, iw);
-                if (sid + d < iw && orid == rid) c += oc;
-            }
-            const bool head = valid && start;
-            float *cell = mine + my_line * lstride + pad + (x < 0 ? 0 : x);
-            for (int q = 0; q < nsub; ++q)
-                if (head && hq == q) *cell = (float)((double)*cell + c);
-        }
+        inject_line(F, line_in_field, injector, iw, sid, fsel, off, vals, line_start, line_count, pts, val,
+                    mine + my_line * lstride + pad);
         __syncthreads();
     }
 
@@ -741,7 +775,11 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
             else slide1<SEG, false>(ln, s0, T, alpha, 1.0, sb);
         }
         __syncthreads();
-        if (act) stage_store(ln, s0, myseg, full, sb);
+        // an opaque copy of the lane's element count keeps the compiler from hoisting the
+        // kSegArr store predicates of the partial case out of the loop (into spilled SGPRs)
+        int nst = myseg;
+        asm volatile("" : "+v"(nst));
+        if (act) stage_store(ln, s0, nst, full, sb);
         if (line_ok)
             for (int e = sid; e < nclr; e += lpl) ln[L + e] = 0.f;
         __syncthreads();
@@ -769,6 +807,292 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     } else {
         tile_copy<false>(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
         tile_copy<false>(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32C prefix pass.  The same n-fold tailed box filter (interpolation.py:484-533), computed
+// on FP64 prefix sums of the line held in LDS.  With P_m = sum of the line's elements
+// before position m (zero padding outside the line), one iteration is
+//   o_k = (1 - alpha) (P_{k+T+1} - P_{k-T}) + alpha (P_{k+T+2} - P_{k-T-1})
+// (the window of 2T+1 terms plus the two tail terms).  Each lane owns SEG consecutive
+// positions: it chains the FP64 prefix of its outputs, the line's lanes scan their segment
+// totals, and the lanes store the next iteration's P = base + chain.  Nothing is rounded to
+// float32 between iterations; the last iteration writes float32 outputs into the line's
+// tile.  5 FP64 operations per element and iteration, no conversions.
+//
+// LDS per (field, line): prefix_line_stride(lpl, SEG, T) doubles holding positions
+// -T-1 .. lpl*SEG+T+1 (position m at index m+T+1): zero for m <= 0, P_L for m >= L.  The
+// float32 tile of the line (tile IO, injection, results) aliases the same storage.  After
+// all lines: one double per wave for the segment-total scan of lines wider than a wave.
+
+// Segment lengths the prefix pass is instantiated for; odd, so the 8-byte LDS lane stride is
+// conflict-free.
+#define FB_PFX_SEGS(X) X(15) X(17) X(19) X(21) X(23)
+constexpr int kPfxBlk = 4;   // prefix reads in flight per stream
+
+// x + (x of the lane CTRL selects), as a DPP move of both halves; lanes whose source is
+// outside the row, and rows outside ROWMASK, add 0
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_add(double x)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROWMASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROWMASK, 0xf, false);
+    return x + __hiloint2double(hi, lo);
+}
+
+// Exclusive scan of the lanes' segment totals t along each line (lpl lanes of consecutive
+// threads) and the line total.  Lines of 16 or more lanes scan with DPP row shifts and row
+// broadcasts (a row is 16 lanes), shorter ones with shuffles.  For lines of several waves it
+// contains the workgroup barrier that ends the read phase; every thread of the workgroup
+// calls it.
+__device__ __forceinline__ void line_scan(double t, int sid, int lpl, double *wtot, double &base, double &total)
+{
+    const int w = lpl < 64 ? lpl : 64;
+    const int li = sid & (w - 1);
+    double inc = t;
+    if (w >= 16) {
+        inc = dpp_add<0x111, 0xf>(inc);   // row_shr:1
+        inc = dpp_add<0x112, 0xf>(inc);   // row_shr:2
+        inc = dpp_add<0x114, 0xf>(inc);   // row_shr:4
+        inc = dpp_add<0x118, 0xf>(inc);   // row_shr:8
+        if (w >= 32) inc = dpp_add<0x142, 0xa>(inc);   // row_bcast:15 into rows 1, 3
+        if (w >= 64) inc = dpp_add<0x143, 0xc>(inc);   // row_bcast:31 into rows 2, 3
+    } else {
+        for (int d = 1; d < w; d <<= 1) {
+            const double o = __shfl_up(inc, d, w);
+            if (li >= d) inc += o;
+        }
+    }
+    const double ex = inc - t;
+    if (lpl > 64) {
+        // lines of several waves: wave totals through LDS
+        if ((threadIdx.x & 63) == 63) wtot[threadIdx.x >> 6] = inc;
+        __syncthreads();
+        const int wv = sid >> 6;
+        const int w0 = (int)(threadIdx.x >> 6) - wv;   // first wave of the line
+        double pre = 0.0, tot = 0.0;
+        for (int k = 0; k < (lpl >> 6); ++k) {
+            const double x = wtot[w0 + k];
+            pre += k < wv ? x : 0.0;
+            tot += x;
+        }
+        base = pre + ex;
+        total = tot;
+    } else {
+        base = ex;
+        total = __shfl(inc, w - 1, w);
+    }
+}
+
+// Orders the LDS accesses of one phase of the prefix pass before the next: a workgroup
+// barrier when a line spans several waves; a line within one wave needs only program order
+// (a wave's LDS instructions execute in order), kept by a compiler fence.
+__device__ __forceinline__ void line_sync(int lpl)
+{
+    if (lpl > 64) __syncthreads();
+    else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// One iteration over the lane's segment: pl[j] = P_{s+j-T-1}, pr[j] = P_{s+j+T+1} for
+// j = 0..SEG.  OUT: o_j = c1 D_j + c2 E_j into res[]; else the chain
+// q[j+1] = q[j] + o_j, and q[nvp] is returned (the partial lane's total).
+template <int SEG, bool OUT>
+__device__ __forceinline__ double prefix_step(const double *pl, const double *pr, double c1, double c2, int nvp,
+                                              double (&q)[SEG + 1], float (&res)[SEG])
+{
+    constexpr int NB = (SEG + kPfxBlk - 1) / kPfxBlk;
+    double la = pl[0], ra = pr[0];
+    double lc[kPfxBlk], rc[kPfxBlk], ln[kPfxBlk], rn[kPfxBlk];
+#pragma unroll
+    for (int u = 0; u < kPfxBlk; ++u) {
+        lc[u] = pl[1 + u];
+        rc[u] = pr[1 + u];
+    }
+    double snap = 0.0;
+    q[0] = 0.0;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        if (b + 1 < NB) {
+#pragma unroll
+            for (int u = 0; u < kPfxBlk; ++u) {
+                const int j = (b + 1) * kPfxBlk + u;
+                if (j < SEG) {
+                    ln[u] = pl[1 + j];
+                    rn[u] = pr[1 + j];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPfxBlk; ++u) {
+            const int j = b * kPfxBlk + u;
+            if (j < SEG) {
+                const double D = ra - lc[u];   // P_{k+T+1} - P_{k-T}
+                const double E = rc[u] - la;   // P_{k+T+2} - P_{k-T-1}
+                if (OUT) {
+                    res[j] = (float)fma(c1, D, c2 * E);
+                } else {
+                    q[j + 1] = fma(c1, D, fma(c2, E, q[j]));
+                }
+                la = lc[u];
+                ra = rc[u];
+            }
+        }
+        if (b + 1 < NB) {
+#pragma unroll
+            for (int u = 0; u < kPfxBlk; ++u) {
+                lc[u] = ln[u];
+                rc[u] = rn[u];
+            }
+        }
+    }
+    if (!OUT) {
+        // q[nvp] with the wave-uniform nvp: scalar branches, no per-element selects
+        switch (nvp) {
+#define FB_SNAP(J) case J: if (J <= SEG) snap = q[J <= SEG ? J : 0]; break;
+            FB_SNAP(1) FB_SNAP(2) FB_SNAP(3) FB_SNAP(4) FB_SNAP(5) FB_SNAP(6) FB_SNAP(7) FB_SNAP(8)
+            FB_SNAP(9) FB_SNAP(10) FB_SNAP(11) FB_SNAP(12) FB_SNAP(13) FB_SNAP(14) FB_SNAP(15) FB_SNAP(16)
+            FB_SNAP(17) FB_SNAP(18) FB_SNAP(19) FB_SNAP(20) FB_SNAP(21) FB_SNAP(22) FB_SNAP(23) FB_SNAP(24)
+#undef FB_SNAP
+        default: break;
+        }
+    }
+    return snap;
+}
+
+template <bool INJECT, int SEG>
+__global__ __launch_bounds__(kFastBlock, 1) void k_prefix_pass(
+    const FieldDev *__restrict__ fields, const double *__restrict__ offsets, const PassWork *__restrict__ work,
+    const PassField *__restrict__ passes, const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
+    const uint32_t *__restrict__ line_start, const uint32_t *__restrict__ line_count, const double *__restrict__ pts,
+    const double *__restrict__ val, float *__restrict__ gv, float *__restrict__ gw, float *__restrict__ out,
+    float *__restrict__ wsave, int final_pass)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const PassWork wk = work[blockIdx.x];
+    const FieldDev &F = fields[wk.field];
+    const PassField P = passes[wk.field];
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int half = nthr >> 1;
+    const int fsel = tid >= half;            // 0: v worker, 1: w worker
+    const int ltid = tid - fsel * half;
+    const int L = (int)P.len;
+    const int T = P.T;
+    const int lpl = P.lpl;
+    const int lines_cap = P.lines;
+    const int lstride = prefix_line_stride(lpl, SEG, T);
+    const int fstride = 2 * lstride;         // floats per line of the float32 tiles
+    const int pad = T + 1;                   // index of position 0
+    const int nl = wk.nlines;
+    double *pall = reinterpret_cast<double *>(smem);
+    double *wtot = pall + (size_t)2 * lines_cap * lstride;
+    float *tv = reinterpret_cast<float *>(smem);
+    float *tw = tv + (size_t)lines_cap * fstride;
+    const int my_line = ltid / lpl;
+    const int sid = ltid - my_line * lpl;
+    const bool line_ok = my_line < nl;
+    double *myP = pall + (size_t)(fsel * lines_cap + (line_ok ? my_line : 0)) * lstride;
+    float *myX = reinterpret_cast<float *>(myP);
+    const double off = offsets[wk.field];
+
+    if (INJECT) {
+        float4 *z = reinterpret_cast<float4 *>(smem);
+        const int nz = (2 * lines_cap * fstride) >> 2;
+        for (int i = tid; i < nz; i += nthr) z[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        __syncthreads();
+        const int iw = lpl < 64 ? lpl : 64;
+        const int64_t line_in_field = wk.base / F.size[0] + my_line;
+        inject_line(F, line_in_field, line_ok && sid < iw, iw, sid, fsel, off, vals, line_start, line_count, pts, val,
+                    myX);
+    } else {
+        tile_copy<true>(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, tv, fstride);
+        tile_copy<true>(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, tw, fstride);
+    }
+    __syncthreads();
+
+    // lane classes: nfull lanes of SEG positions, then (when L mod SEG != 0) one partial lane
+    // of nvp positions, then lanes past the line end
+    const int s0 = sid * SEG;
+    const int nfull = L / SEG;
+    const int nvp = L - nfull * SEG;
+    const int nv = !line_ok ? 0 : (sid < nfull ? SEG : (sid == nfull ? nvp : 0));
+    const bool partial = line_ok && nvp > 0 && sid == nfull;
+    double q[SEG + 1];
+    float res[SEG];
+    double base, total;
+
+    // prefix of the input line
+    q[0] = 0.0;
+#pragma unroll
+    for (int j = 0; j < SEG; ++j) {
+        const float xv = j < nv ? myX[s0 + j] : 0.f;
+        q[j + 1] = q[j] + (double)xv;
+    }
+    line_scan(q[SEG], sid, lpl, wtot, base, total);
+    line_sync(lpl);   // the float32 reads above end before the prefix stores alias them
+    if (line_ok) {
+#pragma unroll
+        for (int j = 0; j < SEG; ++j) myP[pad + s0 + j] = base + q[j];
+        for (int e = sid; e < pad; e += lpl) myP[e] = 0.0;
+        for (int e = sid; e < T + 2; e += lpl) myP[pad + lpl * SEG + e] = total;
+    }
+    line_sync(lpl);
+
+    const double alpha = P.alpha;
+    const int niter = F.num_iter;
+    const double *pl = myP + s0;                 // P_{s0-T-1}
+    const double *pr = myP + s0 + 2 * T + 2;     // P_{s0+T+1}
+    for (int it = 0; it < niter; ++it) {
+        const bool last = it + 1 == niter;
+        const bool grp = (it + 1) % P.sgroup == 0;
+        const double sc = last ? P.slast : (grp ? P.sgscale : 1.0);
+        const double c1 = nv > 0 ? sc * (1.0 - alpha) : 0.0;
+        const double c2 = nv > 0 ? sc * alpha : 0.0;
+        if (!last) {
+            const double snap = prefix_step<SEG, false>(pl, pr, c1, c2, nvp, q, res);
+            line_scan(partial ? snap : q[SEG], sid, lpl, wtot, base, total);
+            line_sync(lpl);
+            if (line_ok) {
+#pragma unroll
+                for (int j = 0; j < SEG; ++j) myP[pad + s0 + j] = base + q[j];
+                // the partial lane's chain ran on past the line end: its positions after L
+                // hold P_L (program order after its own stores above)
+                if (partial)
+                    for (int e = nvp + 1; e < SEG; ++e) myP[pad + s0 + e] = total;
+                for (int e = sid; e < T + 2; e += lpl) myP[pad + lpl * SEG + e] = total;
+            }
+            line_sync(lpl);
+        } else {
+            prefix_step<SEG, true>(pl, pr, c1, c2, nvp, q, res);
+            line_sync(lpl);
+            if (line_ok) {
+#pragma unroll
+                for (int j = 0; j < SEG; ++j) myX[s0 + j] = res[j];
+            }
+            __syncthreads();   // the tile IO below spans all lines
+        }
+    }
+
+    if (final_pass) {
+        // masked quotient + offset (interpolation.py:367, :428-430), in place in the v tiles
+        const double thr = F.thr_fast;
+        for (int l = 0; l < nl; ++l)
+            for (int j = tid; j < L; j += nthr) {
+                const int o = l * fstride + j;
+                const float wf = tw[o];
+                const double w = (double)wf, v = (double)tv[o];
+                const double r0 = (double)__builtin_amdgcn_rcpf(wf);
+                double qq = v * r0;
+                qq = fma(r0, fma(-qq, w, v), qq);
+                tv[o] = (w < thr) ? __builtin_nanf("") : (float)(qq + off);
+            }
+        __syncthreads();
+        tile_copy<false>(out + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, tv, fstride);
+        if (wsave) tile_copy<false>(wsave + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, tw, fstride);
+    } else {
+        tile_copy<false>(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, tv, fstride);
+        tile_copy<false>(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, tw, fstride);
     }
 }
 
@@ -1046,6 +1370,9 @@ int fast_pass_init()
 #define FB_FN(S) (const void *)k_fast_pass<true, S>, (const void *)k_fast_pass<false, S>,
             FB_FAST_SEGS(FB_FN)
 #undef FB_FN
+#define FB_FN(S) (const void *)k_prefix_pass<true, S>, (const void *)k_prefix_pass<false, S>,
+            FB_PFX_SEGS(FB_FN)
+#undef FB_FN
         };
         for (const void *fn : fns) {
             err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, fast_pass_max_lds());
@@ -1060,8 +1387,18 @@ int fast_pass_init()
     return FB_OK;
 }
 
-bool fast_pass_seg_supported(int seg)
+bool fast_pass_seg_supported(int kind, int seg)
 {
+    if (kind == kPassPrefix) {
+        switch (seg) {
+#define FB_CASE(S) case S:
+            FB_PFX_SEGS(FB_CASE)
+#undef FB_CASE
+            return true;
+        default:
+            return false;
+        }
+    }
     switch (seg) {
 #define FB_CASE(S) case S:
         FB_FAST_SEGS(FB_CASE)
@@ -1072,35 +1409,42 @@ bool fast_pass_seg_supported(int seg)
     }
 }
 
-int launch_fast_pass(bool inject, int seg, const FieldDev *d_fields, const double *d_offsets, const PassWork *d_work,
-                     int nwork, const PassField *d_pass, int threads, size_t lds_bytes, const uint32_t *d_keys,
-                     const uint32_t *d_vals, const uint32_t *d_line_start, const uint32_t *d_line_count,
-                     const double *d_pts, const double *d_val, float *d_v, float *d_w, float *d_out, float *d_wsave,
-                     int final_pass, hipStream_t s)
+int launch_fast_pass(int kind, bool inject, int seg, const FieldDev *d_fields, const double *d_offsets,
+                     const PassWork *d_work, int nwork, const PassField *d_pass, int threads, size_t lds_bytes,
+                     const uint32_t *d_keys, const uint32_t *d_vals, const uint32_t *d_line_start,
+                     const uint32_t *d_line_count, const double *d_pts, const double *d_val, float *d_v, float *d_w,
+                     float *d_out, float *d_wsave, int final_pass, hipStream_t s)
 {
     if (nwork <= 0) return FB_OK;
     if (threads != 128 && threads != 256 && threads != kFastBlock) {
         set_error("fast pass: unsupported workgroup size");
         return FB_ERR_UNSUPPORTED;
     }
-    switch (seg) {
-#define FB_LAUNCH(S)                                                                                               \
-    case S:                                                                                                        \
-        if (inject)                                                                                                \
-            hipLaunchKernelGGL((k_fast_pass<true, S>), dim3(nwork), dim3(threads), lds_bytes, s, d_fields,         \
-                               d_offsets, d_work, d_pass, d_keys, d_vals, d_line_start, d_line_count, d_pts, d_val, \
-                               d_v, d_w, d_out, d_wsave, final_pass);                                              \
-        else                                                                                                       \
-            hipLaunchKernelGGL((k_fast_pass<false, S>), dim3(nwork), dim3(threads), lds_bytes, s, d_fields,        \
-                               d_offsets, d_work, d_pass, d_keys, d_vals, d_line_start, d_line_count, d_pts, d_val, \
-                               d_v, d_w, d_out, d_wsave, final_pass);                                              \
-        break;
-        FB_FAST_SEGS(FB_LAUNCH)
-#undef FB_LAUNCH
-    default:
+    if (!fast_pass_seg_supported(kind, seg)) {
         set_error("fast pass: unsupported segment length " + std::to_string(seg));
         return FB_ERR_UNSUPPORTED;
     }
+#define FB_LAUNCH(K, S)                                                                                            \
+    case S:                                                                                                        \
+        if (inject)                                                                                                \
+            hipLaunchKernelGGL((K<true, S>), dim3(nwork), dim3(threads), lds_bytes, s, d_fields, d_offsets, d_work, \
+                               d_pass, d_keys, d_vals, d_line_start, d_line_count, d_pts, d_val, d_v, d_w, d_out,   \
+                               d_wsave, final_pass);                                                               \
+        else                                                                                                       \
+            hipLaunchKernelGGL((K<false, S>), dim3(nwork), dim3(threads), lds_bytes, s, d_fields, d_offsets,        \
+                               d_work, d_pass, d_keys, d_vals, d_line_start, d_line_count, d_pts, d_val, d_v, d_w,  \
+                               d_out, d_wsave, final_pass);                                                        \
+        break;
+#define FB_LAUNCH_SLIDE(S) FB_LAUNCH(k_fast_pass, S)
+#define FB_LAUNCH_PFX(S) FB_LAUNCH(k_prefix_pass, S)
+    if (kind == kPassPrefix) {
+        switch (seg) { FB_PFX_SEGS(FB_LAUNCH_PFX) default: break; }
+    } else {
+        switch (seg) { FB_FAST_SEGS(FB_LAUNCH_SLIDE) default: break; }
+    }
+#undef FB_LAUNCH_PFX
+#undef FB_LAUNCH_SLIDE
+#undef FB_LAUNCH
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }

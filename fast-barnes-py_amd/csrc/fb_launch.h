@@ -45,8 +45,9 @@ int launch_mask(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfie
 // fb_kernels.hip -- FP32C path
 int fast_pass_max_lds();
 int fast_pass_init();
-bool fast_pass_seg_supported(int seg);
-int launch_fast_pass(bool inject, int seg, const FieldDev *d_fields, const double *d_offsets, const PassWork *d_work,
+bool fast_pass_seg_supported(int kind, int seg);
+// kind: kPassSlide (k_fast_pass) or kPassPrefix (k_prefix_pass)
+int launch_fast_pass(int kind, bool inject, int seg, const FieldDev *d_fields, const double *d_offsets, const PassWork *d_work,
                      int nwork, const PassField *d_pass, int threads, size_t lds_bytes, const uint32_t *d_keys,
                      const uint32_t *d_vals, const uint32_t *d_line_start, const uint32_t *d_line_count,
                      const double *d_pts, const double *d_val, float *d_v, float *d_w, float *d_out,

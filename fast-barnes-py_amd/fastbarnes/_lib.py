@@ -79,7 +79,9 @@ SIGNATURES = {
     'fb_plan_forward': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     'fb_plan_backward': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     'fb_plan_autotune': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
-    'fb_plan_pass_groups': (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _i32p, ctypes.c_int,
+    'fb_plan_choose_pass': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int]),
+    'fb_plan_pass_groups': (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _i32p, _i32p, ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_int)]),
     'fb_plan_set_timing': (ctypes.c_int, [_vp, ctypes.c_int]),
     'fb_plan_stage_times': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int,

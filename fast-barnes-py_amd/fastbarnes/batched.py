@@ -118,19 +118,28 @@ class BarnesBatch:
                                      stream))
 
     def autotune(self, d_pts, d_val, d_out, stream):
-        """Times the segment-length variants of every line pass on these buffers and keeps
-        the fastest per axis and line length (FP32C plans).  d_out is left undefined."""
+        """Times the pass kinds, segment lengths and workgroup sizes of every line pass on these
+        buffers and keeps the fastest per axis and line length (FP32C plans).  d_out is left
+        undefined."""
         check(lib().fb_plan_autotune(self._plan, d_pts, d_val, d_out, stream))
 
+    def choose_pass(self, axis, length, kind, seg, threads=256):
+        """Forces the pass along `axis` for lines of `length`: kind 'slide' or 'prefix', the
+        segment length (an instantiated one) and the workgroup size (FP32C plans)."""
+        k = {'slide': 0, 'prefix': 1}[kind]
+        check(lib().fb_plan_choose_pass(self._plan, int(axis), int(length), k, int(seg), int(threads)))
+
     def pass_groups(self, axis):
-        """[(segment length, workgroup threads, work items)] of the launches of the pass
-        along `axis`."""
+        """[(kind, segment length, workgroup threads, work items)] of the launches of the pass
+        along `axis`; kind is 'slide' (float32 lines, FP64 window sums) or 'prefix' (FP64
+        prefix sums)."""
         seg = (ctypes.c_int32 * 32)()
         thr = (ctypes.c_int32 * 32)()
         items = (ctypes.c_int32 * 32)()
+        kind = (ctypes.c_int32 * 32)()
         n = ctypes.c_int()
-        check(lib().fb_plan_pass_groups(self._plan, axis, seg, thr, items, 32, ctypes.byref(n)))
-        return [(seg[i], thr[i], items[i]) for i in range(min(n.value, 32))]
+        check(lib().fb_plan_pass_groups(self._plan, axis, seg, thr, items, kind, 32, ctypes.byref(n)))
+        return [(('slide', 'prefix')[kind[i]], seg[i], thr[i], items[i]) for i in range(min(n.value, 32))]
 
     def set_timing(self, enable):
         check(lib().fb_plan_set_timing(self._plan, int(bool(enable))))

@@ -171,18 +171,29 @@ int fb_plan_forward(fb_plan *plan, const double *d_pts, const double *d_val, flo
 int fb_plan_backward(fb_plan *plan, const double *d_pts, const double *d_val, const float *d_out,
                      const float *d_wsave, const float *d_gout, double *d_gval, double *d_gpts, void *stream);
 
+/* Kinds of FP32C line pass (fb_plan_pass_groups): the sliding-window pass (float32 lines in
+ * LDS, FP64 window sums) and the prefix pass (FP64 prefix sums of the lines in LDS). */
+#define FB_PASS_SLIDE 0
+#define FB_PASS_PREFIX 1
+
 /* Autotuner of the FP32C line passes (no reference counterpart: the reference has no
- * GPU tiling).  For every axis and line length of the plan it times each segment length
- * (elements per lane; it sets the lanes per line and the lines per workgroup) and each
- * workgroup size (128, 256 or 512 threads, LDS scaled to keep 16 waves per CU) on the plan's
- * own buffers and keeps the fastest; untuned plans use a cost model.  Leaves d_out
- * undefined: execute again afterwards.  A no-op for FP64 plans. */
+ * GPU tiling).  For every axis and line length of the plan it times each pass kind, segment
+ * length (elements per lane; it sets the lanes per line and the lines per workgroup) and
+ * workgroup size (128, 256 or 512 threads) on the plan's own buffers and keeps the fastest;
+ * untuned plans use a cost model.  Leaves d_out undefined: execute again afterwards.  A
+ * no-op for FP64 plans. */
 int fb_plan_autotune(fb_plan *plan, const double *d_pts, const double *d_val, float *d_out, void *stream);
 
-/* The launches of the fast pass along `axis`: segment length, workgroup size (threads) and
- * work items of each. */
-int fb_plan_pass_groups(fb_plan *plan, int axis, int32_t *seg, int32_t *threads, int32_t *items, int max_groups,
-                        int *ngroups);
+/* Forces the kind (FB_PASS_SLIDE or FB_PASS_PREFIX), segment length and workgroup size
+ * (128, 256 or 512 threads) of the fast pass along `axis` for the fields whose lines along it
+ * have length `len`, and rebuilds the plan's passes; for manual tuning and tests (no
+ * reference counterpart).  FP32C plans only. */
+int fb_plan_choose_pass(fb_plan *plan, int axis, int64_t len, int kind, int seg, int threads);
+
+/* The launches of the fast pass along `axis`: segment length, workgroup size (threads), work
+ * items and kind (FB_PASS_SLIDE or FB_PASS_PREFIX) of each; any output array may be NULL. */
+int fb_plan_pass_groups(fb_plan *plan, int axis, int32_t *seg, int32_t *threads, int32_t *items, int32_t *kind,
+                        int max_groups, int *ngroups);
 
 /* Per-stage device time (ms) of the last fb_plan_execute run with timing enabled,
  * measured with HIP events recorded on the execution stream.  names receives a

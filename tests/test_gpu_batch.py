@@ -130,8 +130,9 @@ def test_autotune_keeps_results_and_reports_groups():
         batch.autotune(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr)
         for axis in range(len(dim_group[0]['size'])):
             groups = batch.pass_groups(axis)
-            assert groups and all(seg in (25, 29, 33, 37, 41, 45, 49) and thr in (128, 256, 512) and n > 0
-                                  for seg, thr, n in groups)
+            segs = {'slide': (25, 29, 33, 37, 41, 45, 49), 'prefix': (15, 17, 19, 21, 23)}
+            assert groups and all(seg in segs[kind] and thr in (128, 256, 512) and n > 0
+                                  for kind, seg, thr, n in groups)
         batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr, graph=True)
         s.synchronize()
         after = d_out.numpy()
@@ -159,3 +160,69 @@ def test_full_size_batch_matches_single_fields():
             ref = barnes(c['pts'], c['val'], c['sigma'], c['x0'], c['step'], c['size'], num_iter=c['num_iter'],
                          max_dist=c['max_dist'], precision='fp32c')
             _bitwise(o, ref)
+
+
+@pytest.mark.parametrize('seg', [15, 19, 23])
+def test_prefix_pass_matches_fp64(seg):
+    """The prefix pass (FP64 prefix sums of the lines in LDS; an autotuner candidate), forced
+    on every axis through choose_pass: 2D fields with lines of one wave (L <= 64*seg), of
+    several waves and with a partial last lane, a 3D field and a 1D field, against the FP64
+    path (same NaN mask, RMSE <= 1e-5 of the value range) and the sliding pass (within
+    float32 rounding of each other); deterministic across runs and graph replays."""
+    from fastbarnes import hip, workloads
+    from fastbarnes.batched import BarnesBatch
+    rng = np.random.default_rng(seg)
+    cases = []
+    for W, n, sigma, it in ((2400, 3490, 1.0, 4), (600, 2000, 1.7, 6), (150, 700, 0.9, 3)):
+        pts = workloads.stations_2d(n, W + seg)
+        cases.append(dict(pts=pts, val=workloads.values_sinusoids(pts, W), sigma=sigma, x0=np.array([-26.0, 34.0]),
+                          step=75.0 / W, size=(W, W // 2), num_iter=it, max_dist=3.5))
+    c3 = workloads.config4(seed=seg, n=3000, size=(64, 48, 24))
+    cases.append(dict(pts=c3['pts'], val=c3['val'], sigma=c3['sigma'], x0=c3['x0'], step=c3['step'],
+                      size=c3['size'], num_iter=4, max_dist=3.5))
+    p1 = np.sort(rng.uniform(0.0, 30.0, 300))
+    cases.append(dict(pts=p1, val=np.sin(p1 / 3.0) + rng.normal(0, 0.1, 300), sigma=0.8, x0=0.0, step=0.01,
+                      size=(3000,), num_iter=5, max_dist=3.5))
+    for c in cases:
+        dim = len(c['size'])
+        pts2 = c['pts'].reshape(len(c['val']), -1)
+        ref64 = BarnesBatch([_desc(c)], precision='fp64').interpolate([pts2], [c['val']])[0]
+        slide = BarnesBatch([_desc(c)], precision='fp32c').interpolate([pts2], [c['val']])[0]
+        batch = BarnesBatch([_desc(c)], precision='fp32c')
+        for a in range(dim):
+            try:
+                batch.choose_pass(a, c['size'][a], 'prefix', seg, 256)
+            except RuntimeError:   # lines of more than 128 lanes: 512-thread workgroups
+                batch.choose_pass(a, c['size'][a], 'prefix', seg, 512)
+            assert all(k == 'prefix' and sg == seg for k, sg, _, _ in batch.pass_groups(a))
+        d_pts, d_val = hip.DeviceArray.from_numpy(np.ascontiguousarray(pts2)), hip.DeviceArray.from_numpy(c['val'])
+        d_out = hip.DeviceArray((batch.total_out,), np.float32)
+        s = hip.Stream()
+        batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr)
+        s.synchronize()
+        r = d_out.numpy().reshape(ref64.shape)
+        for graph in (True, True, False):
+            batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr, graph=graph)
+            s.synchronize()
+            _bitwise(d_out.numpy().reshape(ref64.shape), r)
+        np.testing.assert_array_equal(np.isnan(r), np.isnan(ref64))
+        ok = ~np.isnan(ref64)
+        assert ok.mean() > 0.2
+        rng_v = float(np.ptp(c['val']))
+        d = r[ok].astype(np.float64) - ref64[ok]
+        assert float(np.sqrt(np.mean(d * d))) <= 1e-5 * rng_v, (c['size'], seg)
+        assert float(np.max(np.abs(r[ok] - slide[ok]) / (np.abs(slide[ok]) + 1.0))) < 1e-5
+
+
+def test_choose_pass_rejects_bad_choices():
+    from fastbarnes.batched import BarnesBatch
+    c = _fields(3)[0]
+    batch = BarnesBatch([_desc(c)], precision='fp32c')
+    with pytest.raises(RuntimeError, match='segment length'):
+        batch.choose_pass(0, c['size'][0], 'prefix', 41, 256)
+    with pytest.raises(RuntimeError, match='no field'):
+        batch.choose_pass(0, c['size'][0] + 1, 'slide', 41, 256)
+    with pytest.raises(RuntimeError, match='threads'):
+        batch.choose_pass(0, c['size'][0], 'slide', 41, 64)
+    with pytest.raises(RuntimeError, match='FP32C'):
+        BarnesBatch([_desc(c)], precision='fp64').choose_pass(0, c['size'][0], 'slide', 41, 256)
