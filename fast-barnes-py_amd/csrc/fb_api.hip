@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <limits>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -474,6 +475,16 @@ namespace {
 
 bool finite_pos(double x) { return std::isfinite(x) && x > 0.0; }
 
+// The smallest float32 t with (double)t >= thr, so that for every float32 w the FP64
+// comparison (double)w < thr equals the float32 comparison w < t.
+float float_threshold(double thr)
+{
+    if (std::isnan(thr)) return thr;
+    float t = (float)thr;
+    if ((double)t < thr) t = std::nextafter(t, std::numeric_limits<float>::infinity());
+    return t;
+}
+
 // validates one field and fills its device descriptor (kernel parameters included)
 int make_field(const fb_field &f, FieldDev &d, int fast)
 {
@@ -519,6 +530,7 @@ int make_field(const fb_field &f, FieldDev &d, int fast)
     }
     d.thr = threshold(d.tail, f.dim, d.sigma, d.step, K, f.num_iter, d.alpha, f.max_dist_weight);
     d.thr_fast = d.thr * prod_scale;
+    d.thr_fast_f = float_threshold(d.thr_fast);
     return FB_OK;
 }
 
@@ -1459,6 +1471,7 @@ int fb_convolve_grid(int tail, int precision, int dim, const int64_t *size, cons
     }
     F.thr = thr;
     F.thr_fast = thr;
+    F.thr_fast_f = float_threshold(thr);
     Scratch sc;
     sc.device = device;
     int rc;

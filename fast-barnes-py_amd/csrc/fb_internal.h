@@ -30,7 +30,7 @@ struct FieldDev {
     double step[3];
     double sigma[3];
     int32_t T[3];          // half kernel size per axis
-    int32_t pad1;
+    float thr_fast_f;      // smallest float32 >= thr_fast: (double)w < thr_fast iff w < thr_fast_f
     double alpha[3];       // tail value per axis (0 for plain convolution)
     double pscale[3];      // (1/(K+2*alpha))^num_iter per axis, FP32C normalisation
     double thr;            // reference NaN-mask threshold (FP64 path)

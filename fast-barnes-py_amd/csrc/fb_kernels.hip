@@ -427,6 +427,18 @@ constexpr int kBlk = 4;           // add-stream block (LDS loads in flight)
 // discarded by stage_store.  Results replace sb[j] (dead after step j: its value lives on
 // in p), so the register array doubles as the staging buffer of the in-place update.
 //   SCALE: multiply the results by `scale` (last iteration of a pass).
+// Masked FP32C quotient v/w + offset of one node (interpolation.py:367, :428-430): a
+// float32 reciprocal refined by one Newton step on the quotient (~1 ulp of v/w; w is
+// normalised, well inside the float32 range), the offset added in FP64 before the final
+// rounding to float32, and w < thr decided exactly as in FP64 through the float32 threshold.
+__device__ __forceinline__ float fast_quotient(float vf, float wf, double off, float thr_f)
+{
+    const float r = __builtin_amdgcn_rcpf(wf);
+    float q = vf * r;
+    q = fmaf(fmaf(-q, wf, vf), r, q);
+    return wf < thr_f ? __builtin_nanf("") : (float)((double)q + off);
+}
+
 template <int SEG, bool SCALE>
 __device__ __forceinline__ void slide1(const float *__restrict__ lin, int s, int T, double alpha, double scale,
                                        float (&sb)[SEG])
@@ -467,12 +479,14 @@ __device__ __forceinline__ void slide1(const float *__restrict__ lin, int s, int
     }
     double a = (v[0] + v[1]) + (v[2] + v[3]);
     // Each step: output = a + alpha*(left + right tail), then a slides by the entering minus
-    // the leaving term, both exact in FP64.  The scaling (SCALE) iteration forms the output in
-    // FP64; the others round a to float32 and add the tails in float32 (one float32 FMA): the
-    // output is stored as float32 either way, and a float32 FMA is half the cost of an FP64 one.
-    double p = (double)p0;
+    // the leaving term, exactly in FP64.  The output rounds a to float32 and adds the tails
+    // in float32 (one float32 FMA; a float32 operation costs half an FP64 one): it is stored
+    // as float32 anyway, and the tails are a small part of it.  The scaling iteration
+    // multiplies by the float32 scale, whose rounding is the same for the v and w fields of
+    // an axis and cancels in their quotient.
     float pf = p0;
     const float alpha_f = (float)alpha;
+    const float scale_f = (float)scale;
     const float *addp = lin + s + T + 1;
     float cur[kBlk], nxt[kBlk];
 #pragma unroll
@@ -489,17 +503,10 @@ __device__ __forceinline__ void slide1(const float *__restrict__ lin, int s, int
             const int j = b * kBlk + u;
             if (j < kSegArr) {
                 const float svf = sb[j];
-                const double ad = (double)cur[u];
-                const double sv = (double)svf;
-                float r;
-                if (SCALE) {
-                    r = (float)(fma(alpha, p + ad, a) * scale);
-                    p = sv;
-                } else {
-                    r = fmaf(alpha_f, pf + cur[u], (float)a);
-                    pf = svf;
-                }
-                a += ad - sv;
+                float r = fmaf(alpha_f, pf + cur[u], (float)a);
+                if (SCALE) r *= scale_f;
+                pf = svf;
+                a += (double)cur[u] - (double)svf;
                 // the empty asm pins each float32 result at its step; otherwise the compiler
                 // sinks all conversions to the store and keeps kSegArr doubles live
                 asm volatile("" : "+v"(r));
@@ -787,18 +794,11 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
 
     if (final_pass) {
         // masked quotient + offset (interpolation.py:367, :428-430), in place in the v lines
-        // quotient: float32 reciprocal refined by one FP64 Newton step (relative error ~1e-14;
-        // w is normalised, so it lies well inside the float32 range)
-        const double thr = F.thr_fast;
+        const float thr_f = F.thr_fast_f;
         for (int l = 0; l < nl; ++l)
             for (int j = tid; j < L; j += nthr) {
                 const int o = l * lstride + pad + j;
-                const float wf = sw[o];
-                const double w = (double)wf, v = (double)sv[o];
-                const double r0 = (double)__builtin_amdgcn_rcpf(wf);
-                double q = v * r0;
-                q = fma(r0, fma(-q, w, v), q);
-                sv[o] = (w < thr) ? __builtin_nanf("") : (float)(q + off);
+                sv[o] = fast_quotient(sv[o], sw[o], off, thr_f);
             }
         __syncthreads();
         tile_copy<false>(out + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
@@ -1076,16 +1076,11 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_prefix_pass(
 
     if (final_pass) {
         // masked quotient + offset (interpolation.py:367, :428-430), in place in the v tiles
-        const double thr = F.thr_fast;
+        const float thr_f = F.thr_fast_f;
         for (int l = 0; l < nl; ++l)
             for (int j = tid; j < L; j += nthr) {
                 const int o = l * fstride + j;
-                const float wf = tw[o];
-                const double w = (double)wf, v = (double)tv[o];
-                const double r0 = (double)__builtin_amdgcn_rcpf(wf);
-                double qq = v * r0;
-                qq = fma(r0, fma(-qq, w, v), qq);
-                tv[o] = (w < thr) ? __builtin_nanf("") : (float)(qq + off);
+                tv[o] = fast_quotient(tv[o], tw[o], off, thr_f);
             }
         __syncthreads();
         tile_copy<false>(out + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, tv, fstride);

@@ -37,8 +37,9 @@ extern "C" {
 /* FB_PREC_FP64: the reference's own FP64 operation order (injection in point order,
  *   sequential running sums per line, FP64 quotient); float32 results equal the
  *   reference's bit for bit.
- * FP32C: FP32 storage with FP64 accumulation of every window sum, FP64 weight
- *   normalisation and quotient; deterministic, RMSE <= 1e-5 against FP64. */
+ * FP32C: FP32 storage between passes with every window sum exact in FP64 (outputs and the
+ *   quotient formed in float32, the value offset added in FP64, the NaN mask decided as in
+ *   FP64); deterministic, RMSE <= 1e-5 against FP64. */
 #define FB_PREC_FP64 0
 #define FB_PREC_FP32C 1
 
