@@ -636,6 +636,7 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
     if (precision != FB_PREC_FP64 && precision != FB_PREC_FP32C)
         return fail(FB_ERR_INVALID_ARGUMENT, "unknown precision");
     int rc0 = fast_pass_init();
+    if (!rc0) rc0 = field_sort_init();
     if (rc0) return rc0;
     fb_plan *P = new fb_plan();
     FB_HIP_CHECK(hipGetDevice(&P->device));
@@ -742,10 +743,17 @@ int plan_run(fb_plan *P, const double *pts, const double *val, float *out, float
                                 P->d_line_count, s)))
         return rc;
     uint32_t *ks = nullptr, *vs = nullptr;
-    if ((rc = radix_sort_pairs(P->d_keys, P->d_vals, P->d_keys2, P->d_vals2, P->nentries, P->key_bits, P->d_hist,
-                               P->d_scan_tmp, s, &ks, &vs)))
-        return rc;
     if ((rc = scan_exclusive(P->d_line_count, P->d_line_start, P->nxlines, P->d_scan_tmp, s))) return rc;
+    if (P->max_n <= kLocalSortMax) {
+        // every field fits a workgroup's LDS: one launch sorts all of them
+        if ((rc = launch_field_sort(P->d_fields, P->d_pt_prefix, P->nfields, (int)P->max_n, P->d_keys,
+                                    P->d_line_start, P->d_vals2, s)))
+            return rc;
+        vs = P->d_vals2;
+    } else if ((rc = radix_sort_pairs(P->d_keys, P->d_vals, P->d_keys2, P->d_vals2, P->nentries, P->key_bits,
+                                      P->d_hist, P->d_scan_tmp, s, &ks, &vs))) {
+        return rc;
+    }
     P->sorted_vals = vs;
     stage_mark(P, s, "sort");
     if (P->precision == FB_PREC_FP64) {

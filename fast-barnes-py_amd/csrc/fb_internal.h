@@ -11,6 +11,8 @@ namespace fb {
 // Sentinel sort key of a contribution that falls outside the grid
 // (reference: the `continue` of interpolation.py:226,249,283).
 constexpr uint32_t kInvalidKey = 0xFFFFFFFFu;
+// largest field (points) the per-field LDS sort takes (64-bit entries: 64 KiB of LDS)
+constexpr int kLocalSortMax = 8192;
 
 // Per-field descriptor as the kernels see it.  Everything is derived on the host in
 // FP64 from the reference's own formulas, except the value offset, which the

@@ -13,6 +13,11 @@ int64_t radix_hist_elems(int64_t n);
 int radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, int64_t n, int bits,
                      uint32_t *hist, uint32_t *scan_tmp, hipStream_t stream, uint32_t **keys_out,
                      uint32_t **vals_out);
+// per-field sort in LDS (fields of at most kLocalSortMax points): the sorted in-grid points
+// of field f from out_vals[row_start[first x-line of f]] on; the same order as the radix sort
+int field_sort_init();
+int launch_field_sort(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int max_n,
+                      const uint32_t *d_keys, const uint32_t *d_row_start, uint32_t *d_out_vals, hipStream_t stream);
 
 // fb_kernels.hip -- injection front end
 // (min+max)/2 per field; with d_part (offsets_part_elems doubles) large fields are reduced

@@ -1,9 +1,12 @@
-// fb_sort.hip -- exclusive scan and stable LSD radix sort of (key, value) uint32 pairs.
+// fb_sort.hip -- exclusive scan, stable LSD radix sort of (key, value) uint32 pairs, and a
+// per-field sort in LDS for batches of small fields.
 //
 // Used to order the injection contributions by grid cell while keeping, inside a
 // cell, the point order of the input (the order in which the reference's
 // _inject_data_*d loop adds them, interpolation.py:246-270).  Histogram counts use
 // integer LDS atomics only, so the result is deterministic.
+#include <mutex>
+
 #include "fb_internal.h"
 #include "fb_launch.h"
 
@@ -19,6 +22,7 @@ constexpr int kRTile = kThreads * kRItems; // 1024 elements per workgroup of the
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kWaves = kThreads / 64;
+constexpr int kLocalSortThreads = 1024;
 
 // ---- scan ----------------------------------------------------------------
 
@@ -132,6 +136,50 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const uint32_t *__re
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// One workgroup per field: the field's (cell key, point index) pairs sorted in LDS by a
+// bitonic network on the 64-bit composite (equal keys stay in point order, as in the
+// stable radix sort), and the field's points inside the grid written in that order from
+// its first row start on.  Fields of at most kLocalSortMax points.
+__global__ __launch_bounds__(kLocalSortThreads) void k_field_sort(const FieldDev *__restrict__ fields,
+                                                                  const int64_t *__restrict__ pt_prefix,
+                                                                  const uint32_t *__restrict__ keys,
+                                                                  const uint32_t *__restrict__ row_start,
+                                                                  uint32_t *__restrict__ out_vals)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned long long *sk = reinterpret_cast<unsigned long long *>(smem);
+    const int f = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int64_t p0 = pt_prefix[f];
+    const int n = (int)(pt_prefix[f + 1] - p0);
+    int np2 = 1;
+    while (np2 < n) np2 <<= 1;
+    for (int i = tid; i < np2; i += kLocalSortThreads)
+        sk[i] = i < n ? ((unsigned long long)keys[p0 + i] << 32) | (unsigned long long)(uint32_t)(p0 + i) : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= np2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = tid; t < (np2 >> 1); t += kLocalSortThreads) {
+                // the t-th compare-exchange pair (i, i + j) with bit j of i clear
+                const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+                const int l = i + j;
+                const unsigned long long a = sk[i], b = sk[l];
+                const bool up = (i & k) == 0;
+                if ((a > b) == up) {
+                    sk[i] = b;
+                    sk[l] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const uint32_t dst = row_start[fields[f].xline_off];
+    for (int i = tid; i < n; i += kLocalSortThreads) {
+        const unsigned long long e = sk[i];
+        if ((uint32_t)(e >> 32) != kInvalidKey) out_vals[dst + i] = (uint32_t)e;
+    }
+}
+
 }  // namespace
 
 int64_t scan_temp_elems(int64_t n)
@@ -189,6 +237,40 @@ int radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_
     }
     *keys_out = ki;
     *vals_out = vi;
+    return FB_OK;
+}
+
+int local_sort_max() { return kLocalSortMax; }
+
+int field_sort_init()
+{
+    static std::once_flag once;
+    static hipError_t err = hipSuccess;
+    std::call_once(once, [] {
+        err = hipFuncSetAttribute((const void *)k_field_sort, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  kLocalSortMax * (int)sizeof(unsigned long long));
+    });
+    if (err != hipSuccess) {
+        (void)hipGetLastError();
+        set_error(std::string("hipFuncSetAttribute failed: ") + hipGetErrorString(err));
+        return FB_ERR_HIP;
+    }
+    return FB_OK;
+}
+
+int launch_field_sort(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int max_n,
+                      const uint32_t *d_keys, const uint32_t *d_row_start, uint32_t *d_out_vals, hipStream_t stream)
+{
+    if (nfields <= 0) return FB_OK;
+    if (max_n > kLocalSortMax) {
+        set_error("field sort: a field has more than " + std::to_string(kLocalSortMax) + " points");
+        return FB_ERR_UNSUPPORTED;
+    }
+    int np2 = 1;
+    while (np2 < max_n) np2 <<= 1;
+    hipLaunchKernelGGL(k_field_sort, dim3(nfields), dim3(kLocalSortThreads), (size_t)np2 * 8, stream, d_fields,
+                       d_pt_prefix, d_keys, d_row_start, d_out_vals);
+    FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }
 

@@ -226,3 +226,28 @@ def test_choose_pass_rejects_bad_choices():
         batch.choose_pass(0, c['size'][0], 'slide', 41, 64)
     with pytest.raises(RuntimeError, match='FP32C'):
         BarnesBatch([_desc(c)], precision='fp64').choose_pass(0, c['size'][0], 'slide', 41, 256)
+
+
+def test_sort_paths_agree_bitwise():
+    """Plans whose fields all have at most 8192 points sort each field in LDS (one
+    workgroup per field); a plan with a larger field uses the global radix sort.  Both must
+    give the same per-cell point order (the reference's injection order), hence the same
+    FP64 bits: a batch with a 9000-point field (radix) against single-field calls of its
+    small fields (LDS sort), and the large field against the oracle."""
+    from fastbarnes import workloads
+    from fastbarnes.batched import BarnesBatch
+    from fastbarnes.interpolation import barnes
+    fs = _fields(5)[:3]
+    pts = workloads.stations_2d(9000, 77)
+    fs.append(dict(pts=pts, val=workloads.values_sinusoids(pts, 77), sigma=1.1, x0=np.array([-26.0, 34.0]),
+                   step=0.125, size=(600, 300), num_iter=4, max_dist=3.5))
+    batch = BarnesBatch([_desc(c) for c in fs], precision='fp64')
+    outs = batch.interpolate([c['pts'] for c in fs], [c['val'] for c in fs])
+    for c, o in zip(fs, outs):
+        ref = barnes(c['pts'], c['val'], c['sigma'], c['x0'], c['step'], c['size'], num_iter=c['num_iter'],
+                     max_dist=c['max_dist'], precision='fp64')
+        _bitwise(o, ref)
+    from oracle import oracle as orc
+    c = fs[-1]
+    _bitwise(outs[-1], orc.interpolate_convol(c['pts'], c['val'], c['sigma'], c['x0'], c['step'], c['size'],
+                                              num_iter=c['num_iter'], max_dist=c['max_dist']))
