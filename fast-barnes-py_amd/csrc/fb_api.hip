@@ -618,12 +618,12 @@ int build_passes(fb_plan *P)
 
 // One fast pass along `axis` over the whole batch: a launch per segment-length group.
 int run_pass(fb_plan *P, int axis, bool inject, const uint32_t *vals, const double *pts, const double *val,
-             float *out, float *wsave, int final_pass, hipStream_t s)
+             float *out, float *wsave, int final_pass, hipStream_t s, const float *adj_g = nullptr)
 {
     for (const PassGroup &g : P->groups[axis]) {
         int rc = launch_fast_pass(g.kind, inject, g.seg, P->d_fields, P->d_offsets, P->d_work[axis] + g.first, g.count,
                                   P->d_pass[axis], 2 * g.lpf, g.lds, nullptr, vals, P->d_line_start,
-                                  P->d_line_count, pts, val, P->d_v, P->d_w, out, wsave, final_pass, s);
+                                  P->d_line_count, pts, val, P->d_v, P->d_w, out, wsave, final_pass, adj_g, s);
         if (rc) return rc;
     }
     return FB_OK;
@@ -737,22 +737,30 @@ int plan_run(fb_plan *P, const double *pts, const double *val, float *out, float
     P->events.clear();
     P->stage_names.clear();
     stage_mark(P, s, "begin");
+    // Fields of at most kLocalSortMax points: per-row counts by atomics, their scan, then one
+    // workgroup sorts each field in LDS.  Larger fields: the radix sort over the batch, then
+    // the row ranges from the sorted keys (no atomics, no scan).
+    const bool local = P->max_n <= kLocalSortMax;
     if ((rc = launch_zero(P->d_line_count, P->nxlines * 4, s))) return rc;
+    if (!local && (rc = launch_zero(P->d_line_start, P->nxlines * 4, s))) return rc;
     if ((rc = launch_offsets(P->d_fields, P->nfields, P->max_n, val, P->d_off_part, P->d_offsets, s))) return rc;
     if ((rc = launch_point_keys(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints, pts, P->d_keys, P->d_vals,
-                                P->d_line_count, s)))
+                                local ? P->d_line_count : nullptr, s)))
         return rc;
     uint32_t *ks = nullptr, *vs = nullptr;
-    if ((rc = scan_exclusive(P->d_line_count, P->d_line_start, P->nxlines, P->d_scan_tmp, s))) return rc;
-    if (P->max_n <= kLocalSortMax) {
-        // every field fits a workgroup's LDS: one launch sorts all of them
+    if (local) {
+        if ((rc = scan_exclusive(P->d_line_count, P->d_line_start, P->nxlines, P->d_scan_tmp, s))) return rc;
         if ((rc = launch_field_sort(P->d_fields, P->d_pt_prefix, P->nfields, (int)P->max_n, P->d_keys,
                                     P->d_line_start, P->d_vals2, s)))
             return rc;
         vs = P->d_vals2;
-    } else if ((rc = radix_sort_pairs(P->d_keys, P->d_vals, P->d_keys2, P->d_vals2, P->nentries, P->key_bits,
-                                      P->d_hist, P->d_scan_tmp, s, &ks, &vs))) {
-        return rc;
+    } else {
+        if ((rc = radix_sort_pairs(P->d_keys, P->d_vals, P->d_keys2, P->d_vals2, P->nentries, P->key_bits,
+                                   P->d_hist, P->d_scan_tmp, s, &ks, &vs)))
+            return rc;
+        if ((rc = launch_row_ranges(P->d_fields, P->d_cell_prefix, P->nfields, ks, P->nentries, P->d_line_start,
+                                    P->d_line_count, P->nxlines, s)))
+            return rc;
     }
     P->sorted_vals = vs;
     stage_mark(P, s, "sort");
@@ -936,12 +944,12 @@ int fb_plan_backward(fb_plan *P, const double *d_pts, const double *d_val, const
     hipStream_t s = (hipStream_t)stream;
     int rc;
     if ((rc = launch_offsets(P->d_fields, P->nfields, P->max_n, d_val, P->d_off_part, P->d_offsets, s))) return rc;
-    if ((rc = launch_adj_prep(P->d_fields, P->d_cell_prefix, P->nfields, P->ncells, P->d_offsets, d_out, d_wsave,
-                              d_gout, P->d_v, P->d_w, s)))
-        return rc;
-    // the same scaled line passes, reading both fields from HBM on every axis
+    // the same scaled line passes; the first one forms a = g/W and b = -a(f - offset) in its tile
+    // load, so a and b never round-trip through HBM
     for (int a = 0; a < P->dim; ++a)
-        if ((rc = run_pass(P, a, false, nullptr, nullptr, nullptr, nullptr, nullptr, 0, s))) return rc;
+        if ((rc = run_pass(P, a, false, nullptr, nullptr, nullptr, a == 0 ? const_cast<float *>(d_out) : nullptr,
+                           a == 0 ? const_cast<float *>(d_wsave) : nullptr, 0, s, a == 0 ? d_gout : nullptr)))
+            return rc;
     return launch_adj_gather(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints, d_pts, d_val, P->d_offsets,
                              P->d_v, P->d_w, d_gval, d_gpts, s);
 }
@@ -979,7 +987,7 @@ static int time_pass_subset(fb_plan *P, int a, const std::vector<PassWork> &work
         for (const PassGroup &g : fg) {
             rc = launch_fast_pass(g.kind, a == 0, g.seg, P->d_fields, P->d_offsets, dw + g.first, g.count, dp, 2 * g.lpf,
                                   g.lds, nullptr, P->sorted_vals, P->d_line_start, P->d_line_count, pts, val, P->d_v,
-                                  P->d_w, out, nullptr, fin, s);
+                                  P->d_w, out, nullptr, fin, nullptr, s);
             if (rc) break;
         }
     }
@@ -1441,7 +1449,7 @@ int fb_accumulate_lines(int tail, int precision, int64_t nlines, int64_t len, in
         if ((rc = launch_zero(d_w, cells * 4, s))) return rc;
         if ((rc = launch_fast_pass(kPassSlide, false, seg, d_f, d_off, d_work, (int)work.size(), d_pf, 2 * kBlock, lds,
                                    nullptr, nullptr,
-                                   nullptr, nullptr, nullptr, nullptr, d_v, d_w, nullptr, nullptr, 0, s)))
+                                   nullptr, nullptr, nullptr, nullptr, d_v, d_w, nullptr, nullptr, 0, nullptr, s)))
             return rc;
         if ((rc = launch_f32_to_f64(d_v, d_io, cells, 1.0, s))) return rc;
     }
@@ -1535,7 +1543,7 @@ int fb_convolve_grid(int tail, int precision, int dim, const int64_t *size, cons
             for (const PassGroup &g : groups)
                 if ((rc = launch_fast_pass(g.kind, false, g.seg, d_f, d_off, d_work + g.first, g.count, d_pf, 2 * g.lpf,
                                            g.lds, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, d_v, d_w,
-                                           nullptr, nullptr, 0, s)))
+                                           nullptr, nullptr, 0, nullptr, s)))
                     return rc;
             FB_HIP_CHECK(hipStreamSynchronize(s));   // host vectors above go out of scope
         }

@@ -175,6 +175,7 @@ __global__ __launch_bounds__(kBlock) void k_point_keys(const FieldDev *__restric
                                                        uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                                        uint32_t *__restrict__ row_count)
 {
+    // row_count == nullptr: the rows' ranges are taken from the sorted keys instead
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= npoints) return;
     const int f = find_seg(pt_prefix, nfields, i);
@@ -188,7 +189,41 @@ __global__ __launch_bounds__(kBlock) void k_point_keys(const FieldDev *__restric
     }
     const int64_t row = pp.i[2] * F.size[1] + pp.i[1];
     keys[i] = (uint32_t)(F.cell_off + row * F.size[0] + pp.i[0]);
-    atomicAdd(&row_count[F.xline_off + row], 1u);
+    if (row_count) atomicAdd(&row_count[F.xline_off + row], 1u);
+}
+
+// Global x-line (base row) of a sorted key inside the grid.
+__device__ __forceinline__ int64_t key_row(const FieldDev *__restrict__ fields, const int64_t *__restrict__ cell_prefix,
+                                           int nfields, uint32_t k)
+{
+    const FieldDev &F = fields[find_seg(cell_prefix, nfields, (int64_t)k)];
+    return F.xline_off + ((int64_t)k - F.cell_off) / F.size[0];
+}
+
+// Row ranges from the sorted keys: the first entry of a row's run stores its position in
+// row_start, the last one its end in row_end (rows without points keep the zeros they were
+// given).  No atomics, no scan.
+__global__ __launch_bounds__(kBlock) void k_row_bounds(const FieldDev *__restrict__ fields,
+                                                       const int64_t *__restrict__ cell_prefix, int nfields,
+                                                       const uint32_t *__restrict__ keys, int64_t n,
+                                                       uint32_t *__restrict__ row_start, uint32_t *__restrict__ row_end)
+{
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t k = keys[p];
+    if (k == kInvalidKey) return;
+    const int64_t r = key_row(fields, cell_prefix, nfields, k);
+    if (p == 0 || key_row(fields, cell_prefix, nfields, keys[p - 1]) != r) row_start[r] = (uint32_t)p;
+    const uint32_t kn = p + 1 < n ? keys[p + 1] : kInvalidKey;
+    if (kn == kInvalidKey || key_row(fields, cell_prefix, nfields, kn) != r) row_end[r] = (uint32_t)(p + 1);
+}
+
+// row_count[r] = row_end[r] - row_start[r], in place (row_count holds the ends on entry)
+__global__ __launch_bounds__(kBlock) void k_row_counts(const uint32_t *__restrict__ row_start,
+                                                       uint32_t *__restrict__ row_count, int64_t nrows)
+{
+    const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (r < nrows) row_count[r] -= row_start[r];
 }
 
 // The sub-streams of x-line (y, z): for q < 2*nrows, the points of base row (y - dy, z - dz)
@@ -623,6 +658,70 @@ __device__ __forceinline__ void tile_copy(float *__restrict__ g, int64_t ls, int
     }
 }
 
+// Load of the adjoint's first pass (rows, element stride 1): from the forward output f, the
+// saved normalised weights W and dL/df = g of a row tile it forms a = g / W and
+// b = -a (f - offset) (the two fields the adjoint filters; zero where f or g is NaN) straight
+// into the v and w lines, instead of a separate pass writing them to HBM.  float32
+// arithmetic: the quotient with one Newton step, and f - offset as (f - hi) - lo with the
+// offset split into float32 hi + lo (f - hi is exact while f is within a factor 2 of hi).
+// float4 global loads when the rows allow.
+__device__ __forceinline__ void adjoint_ab(float fo, float go, float wv, float hi, float lo, float &a, float &b)
+{
+    a = 0.f;
+    b = 0.f;
+    if (fo == fo && go == go) {
+        const float r = __builtin_amdgcn_rcpf(wv);
+        float q = go * r;
+        q = fmaf(fmaf(-q, wv, go), r, q);
+        a = q;
+        b = -q * ((fo - hi) - lo);
+    }
+}
+
+__device__ __forceinline__ void adjoint_load(const float *__restrict__ f, const float *__restrict__ w,
+                                             const float *__restrict__ g, double off, int64_t ls, int nl, int L,
+                                             float *__restrict__ sa, float *__restrict__ sb, int lstride)
+{
+    const float hi = (float)off;
+    const float lo = (float)(off - (double)hi);
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    if ((L & 3) == 0 && (ls & 3) == 0 && aligned16(f) && aligned16(w) && aligned16(g)) {
+        const int Lq = L >> 2;
+        for (int l = 0; l < nl; ++l) {
+            const float4 *fl = reinterpret_cast<const float4 *>(f + l * ls);
+            const float4 *wl = reinterpret_cast<const float4 *>(w + l * ls);
+            const float4 *gl = reinterpret_cast<const float4 *>(g + l * ls);
+            float *da = sa + l * lstride, *db = sb + l * lstride;
+            for (int q = tid; q < Lq; q += nthr) {
+                const float4 fo = fl[q], go = gl[q], wv = wl[q];
+                float a, b;
+                adjoint_ab(fo.x, go.x, wv.x, hi, lo, a, b);
+                da[4 * q] = a;
+                db[4 * q] = b;
+                adjoint_ab(fo.y, go.y, wv.y, hi, lo, a, b);
+                da[4 * q + 1] = a;
+                db[4 * q + 1] = b;
+                adjoint_ab(fo.z, go.z, wv.z, hi, lo, a, b);
+                da[4 * q + 2] = a;
+                db[4 * q + 2] = b;
+                adjoint_ab(fo.w, go.w, wv.w, hi, lo, a, b);
+                da[4 * q + 3] = a;
+                db[4 * q + 3] = b;
+            }
+        }
+    } else {
+        for (int l = 0; l < nl; ++l) {
+            const float *fl = f + l * ls, *wl = w + l * ls, *gl = g + l * ls;
+            for (int j = tid; j < L; j += nthr) {
+                float a, b;
+                adjoint_ab(fl[j], gl[j], wl[j], hi, lo, a, b);
+                sa[l * lstride + j] = a;
+                sb[l * lstride + j] = b;
+            }
+        }
+    }
+}
+
 // Deterministic fused injection of one line's points into its float32 row (element x at
 // row[x]).  The line's sub-streams (sub_stream: one base row, one corner column),
 // concatenated, are taken in chunks of up to 64 entries by the line's first `iw` lanes
@@ -698,7 +797,7 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     const PassField *__restrict__ passes, const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
     const uint32_t *__restrict__ line_start, const uint32_t *__restrict__ line_count, const double *__restrict__ pts,
     const double *__restrict__ val, float *__restrict__ gv, float *__restrict__ gw, float *__restrict__ out,
-    float *__restrict__ wsave, int final_pass)
+    float *__restrict__ wsave, int final_pass, const float *__restrict__ adj_g)
 {
     constexpr int kSegArr = SEG;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -745,8 +844,13 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
             sv[o] = 0.f;
             sw[o] = 0.f;
         }
-        tile_copy<true>(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
-        tile_copy<true>(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
+        if (adj_g)
+            adjoint_load(out + F.out_off + wk.base, wsave + F.out_off + wk.base, adj_g + F.out_off + wk.base, off,
+                         P.line_stride, nl, L, sv + pad, sw + pad, lstride);
+        else {
+            tile_copy<true>(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
+            tile_copy<true>(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
+        }
     }
     __syncthreads();
 
@@ -966,7 +1070,7 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_prefix_pass(
     const PassField *__restrict__ passes, const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
     const uint32_t *__restrict__ line_start, const uint32_t *__restrict__ line_count, const double *__restrict__ pts,
     const double *__restrict__ val, float *__restrict__ gv, float *__restrict__ gw, float *__restrict__ out,
-    float *__restrict__ wsave, int final_pass)
+    float *__restrict__ wsave, int final_pass, const float *__restrict__ adj_g)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const PassWork wk = work[blockIdx.x];
@@ -1006,8 +1110,13 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_prefix_pass(
         inject_line(F, line_in_field, line_ok && sid < iw, iw, sid, fsel, off, vals, line_start, line_count, pts, val,
                     myX);
     } else {
-        tile_copy<true>(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, tv, fstride);
-        tile_copy<true>(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, tw, fstride);
+        if (adj_g)
+            adjoint_load(out + F.out_off + wk.base, wsave + F.out_off + wk.base, adj_g + F.out_off + wk.base, off,
+                         P.line_stride, nl, L, tv, tw, fstride);
+        else {
+            tile_copy<true>(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, tv, fstride);
+            tile_copy<true>(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, tw, fstride);
+        }
     }
     __syncthreads();
 
@@ -1203,34 +1312,10 @@ __global__ __launch_bounds__(kBlock) void k_naive(const FieldDev *__restrict__ f
 
 // a and b at every node of the batch in the cell layout; zero where the forward result
 // is NaN (masked) or the incoming gradient is NaN, and in the alignment gaps.
-__global__ __launch_bounds__(kBlock) void k_adj_prep(const FieldDev *__restrict__ fields,
-                                                     const int64_t *__restrict__ cell_prefix, int nfields,
-                                                     int64_t ncells, const double *__restrict__ offsets,
-                                                     const float *__restrict__ out, const float *__restrict__ wsave,
-                                                     const float *__restrict__ gout, float *__restrict__ av,
-                                                     float *__restrict__ bv)
-{
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (g >= ncells) return;
-    const int f = find_seg(cell_prefix, nfields, g);
-    const FieldDev &F = fields[f];
-    const int64_t i = g - F.cell_off;
-    float ra = 0.f, rb = 0.f;
-    if (i < F.size[0] * F.size[1] * F.size[2]) {
-        const float fo = out[F.out_off + i];
-        const float go = gout[F.out_off + i];
-        if (fo == fo && go == go) {
-            const double a = (double)go / (double)wsave[F.out_off + i];
-            ra = (float)a;
-            rb = (float)(-a * ((double)fo - offsets[f]));
-        }
-    }
-    av[g] = ra;
-    bv[g] = rb;
-}
-
 // Transposed injection: one thread per point gathers A and Bb at its 2^dim corners.
-// Points dropped by the injection (outside the grid, NaN) get zero gradients.
+// Points dropped by the injection (outside the grid, NaN) get zero gradients.  (Taking the
+// points in the forward's sorted order makes the A, Bb reads local but the point reads and
+// gradient writes scattered; measured slower.)
 __global__ __launch_bounds__(kBlock) void k_adj_gather(const FieldDev *__restrict__ fields,
                                                        const int64_t *__restrict__ pt_prefix, int nfields,
                                                        int64_t npoints, const double *__restrict__ pts,
@@ -1305,6 +1390,21 @@ int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int 
     if (npoints <= 0) return FB_OK;
     hipLaunchKernelGGL(k_point_keys, dim3(nblocks(npoints)), dim3(kBlock), 0, s, d_fields, d_pt_prefix, nfields,
                        npoints, d_pts, d_keys, d_vals, d_row_count);
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
+int launch_row_ranges(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfields, const uint32_t *d_keys,
+                      int64_t n, uint32_t *d_row_start, uint32_t *d_row_count, int64_t nrows, hipStream_t s)
+{
+    if (nrows <= 0) return FB_OK;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_row_bounds, dim3(nblocks(n)), dim3(kBlock), 0, s, d_fields, d_cell_prefix, nfields, d_keys,
+                           n, d_row_start, d_row_count);
+        FB_HIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_row_counts, dim3(nblocks(nrows)), dim3(kBlock), 0, s, (const uint32_t *)d_row_start,
+                       d_row_count, nrows);
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }
@@ -1408,7 +1508,7 @@ int launch_fast_pass(int kind, bool inject, int seg, const FieldDev *d_fields, c
                      const PassWork *d_work, int nwork, const PassField *d_pass, int threads, size_t lds_bytes,
                      const uint32_t *d_keys, const uint32_t *d_vals, const uint32_t *d_line_start,
                      const uint32_t *d_line_count, const double *d_pts, const double *d_val, float *d_v, float *d_w,
-                     float *d_out, float *d_wsave, int final_pass, hipStream_t s)
+                     float *d_out, float *d_wsave, int final_pass, const float *d_adj_g, hipStream_t s)
 {
     if (nwork <= 0) return FB_OK;
     if (threads != 128 && threads != 256 && threads != kFastBlock) {
@@ -1424,11 +1524,11 @@ int launch_fast_pass(int kind, bool inject, int seg, const FieldDev *d_fields, c
         if (inject)                                                                                                \
             hipLaunchKernelGGL((K<true, S>), dim3(nwork), dim3(threads), lds_bytes, s, d_fields, d_offsets, d_work, \
                                d_pass, d_keys, d_vals, d_line_start, d_line_count, d_pts, d_val, d_v, d_w, d_out,   \
-                               d_wsave, final_pass);                                                               \
+                               d_wsave, final_pass, d_adj_g);                                                      \
         else                                                                                                       \
             hipLaunchKernelGGL((K<false, S>), dim3(nwork), dim3(threads), lds_bytes, s, d_fields, d_offsets,        \
                                d_work, d_pass, d_keys, d_vals, d_line_start, d_line_count, d_pts, d_val, d_v, d_w,  \
-                               d_out, d_wsave, final_pass);                                                        \
+                               d_out, d_wsave, final_pass, d_adj_g);                                               \
         break;
 #define FB_LAUNCH_SLIDE(S) FB_LAUNCH(k_fast_pass, S)
 #define FB_LAUNCH_PFX(S) FB_LAUNCH(k_prefix_pass, S)
@@ -1440,17 +1540,6 @@ int launch_fast_pass(int kind, bool inject, int seg, const FieldDev *d_fields, c
 #undef FB_LAUNCH_PFX
 #undef FB_LAUNCH_SLIDE
 #undef FB_LAUNCH
-    FB_HIP_CHECK(hipGetLastError());
-    return FB_OK;
-}
-
-int launch_adj_prep(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfields, int64_t ncells,
-                    const double *d_offsets, const float *d_out, const float *d_wsave, const float *d_gout,
-                    float *d_a, float *d_b, hipStream_t s)
-{
-    if (ncells <= 0) return FB_OK;
-    hipLaunchKernelGGL(k_adj_prep, dim3(nblocks(ncells)), dim3(kBlock), 0, s, d_fields, d_cell_prefix, nfields,
-                       ncells, d_offsets, d_out, d_wsave, d_gout, d_a, d_b);
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }

@@ -25,10 +25,14 @@ int launch_field_sort(const FieldDev *d_fields, const int64_t *d_pt_prefix, int 
 int64_t offsets_part_elems(int nfields, int64_t max_n);
 int launch_offsets(const FieldDev *d_fields, int nfields, int64_t max_n, const double *d_val, double *d_part,
                    double *d_offsets, hipStream_t stream);
-// one (base cell, point) pair per point; per-base-row point counts (row = x-line index)
+// one (base cell, point) pair per point; per-base-row point counts (row = x-line index) by
+// integer atomics unless d_row_count is NULL
 int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
                       const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_row_count,
                       hipStream_t stream);
+// per-base-row start and count from the sorted keys (both arrays zeroed by the caller)
+int launch_row_ranges(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfields, const uint32_t *d_keys,
+                      int64_t n, uint32_t *d_row_start, uint32_t *d_row_count, int64_t nrows, hipStream_t stream);
 
 // fb_kernels.hip -- FP64 path with the reference's operation order
 int launch_exact_inject(const FieldDev *d_fields, const int64_t *d_xline_prefix, int nfields, int64_t nxlines,
@@ -51,17 +55,15 @@ int launch_mask(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfie
 int fast_pass_max_lds();
 int fast_pass_init();
 bool fast_pass_seg_supported(int kind, int seg);
-// kind: kPassSlide (k_fast_pass) or kPassPrefix (k_prefix_pass)
+// kind: kPassSlide (k_fast_pass) or kPassPrefix (k_prefix_pass).  d_adj_g (first pass of the
+// adjoint): the tile load forms a = g/W, b = -a(f - offset) from d_out = f, d_wsave = W, d_adj_g = g.
 int launch_fast_pass(int kind, bool inject, int seg, const FieldDev *d_fields, const double *d_offsets, const PassWork *d_work,
                      int nwork, const PassField *d_pass, int threads, size_t lds_bytes, const uint32_t *d_keys,
                      const uint32_t *d_vals, const uint32_t *d_line_start, const uint32_t *d_line_count,
                      const double *d_pts, const double *d_val, float *d_v, float *d_w, float *d_out,
-                     float *d_wsave, int final_pass, hipStream_t stream);
+                     float *d_wsave, int final_pass, const float *d_adj_g, hipStream_t stream);
 
 // fb_kernels.hip -- adjoint of the FP32C batch
-int launch_adj_prep(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfields, int64_t ncells,
-                    const double *d_offsets, const float *d_out, const float *d_wsave, const float *d_gout,
-                    float *d_a, float *d_b, hipStream_t stream);
 int launch_adj_gather(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
                       const double *d_pts, const double *d_val, const double *d_offsets, const float *d_A,
                       const float *d_Bb, double *d_gval, double *d_gpts, hipStream_t stream);
