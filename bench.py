@@ -276,8 +276,8 @@ def main():
     global AUTOTUNE
     AUTOTUNE = not args.no_autotune
     if args.gpus != 1:
-        print('bench.py measures one GPU only: this build has no multi-GPU path (fields shard across GPUs '
-              'with no communication; see DESIGN.md). Run with --gpus 1.', file=sys.stderr)
+        print('bench.py measures one GPU only: there is no multi-GPU path (a batch would shard by fields '
+              'across GPUs with no communication). Run with --gpus 1.', file=sys.stderr)
         sys.exit(2)
 
     from fastbarnes import hip, _lib

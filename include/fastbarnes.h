@@ -24,7 +24,7 @@ extern "C" {
 #define FB_OK 0
 #define FB_ERR_INVALID_ARGUMENT 1   /* maps to the reference's RuntimeError */
 #define FB_ERR_HIP 2                /* a HIP runtime call failed */
-#define FB_ERR_UNSUPPORTED 3        /* arguments valid for the reference, outside this build's limits */
+#define FB_ERR_UNSUPPORTED 3        /* arguments valid for the reference, outside this library's limits */
 #define FB_ERR_OUT_OF_MEMORY 4
 
 /* ---- methods (the `method` string of barnes(), interpolation.py:65-76,169-199) */
