@@ -15,10 +15,11 @@ one core) on a bounded sample of the same fields.  'workloads' adds the other
 workloads, each with its throughput, pipeline roofline fraction and error against the
 FP64 path: cfg3 (256 uneven 2D fields), cfg4 (3D 512x512x128) and cfg4_batch (32 3D
 fields of 256x256x64), cfg5 (16 fields of 2400x1200, N=100000, forward + adjoint), real
-(the RMSE-versus-iterations curve on the 3490 real stations of tests/golden/qff).
+(the RMSE-versus-iterations curve on the 3490 real stations of tests/golden/qff), rows
+(method='convolution', barnes_S2, naive and radius, each beside the CPU oracle).
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--fields 64] [--cpu-seconds 10]
-                    [--legs cfg3,cfg4,cfg5,real|none] [--leg-steps 10]
+                    [--legs cfg3,cfg4,cfg5,real,rows|none] [--leg-steps 10]
 """
 import argparse
 import json
@@ -229,6 +230,92 @@ def leg_real(steps):
                 max_abs_rmse_diff_fp32c_vs_fp64=diff)
 
 
+def _best_of(fn, reps=3):
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = fn()
+        dt = time.perf_counter() - t0
+        best = dt if best is None or dt < best else best
+    return best, r
+
+
+def _err_vs(out, ref):
+    ok = ~np.isnan(ref) & ~np.isnan(out)
+    d = out[ok].astype(np.float64) - ref[ok].astype(np.float64)
+    return dict(rmse=float(np.sqrt(np.mean(d * d))) if d.size else 0.0,
+                maxabs=float(np.max(np.abs(d))) if d.size else 0.0,
+                mask_mismatch=int((np.isnan(ref) != np.isnan(out)).sum()))
+
+
+def leg_rows(cfgs, steps):
+    """The other entry points of the path, each timed on the GPU beside the CPU oracle on a
+    bounded sample, with its error against the oracle there:
+    * method='convolution' (alpha = 0): FP32C batch of the headline fields (graph replay) and
+      the FP64 drop-in of one field (numpy in and out);
+    * barnes_S2 (Lambert projection, convolution on the Lambert grid, resampling) on the 3490
+      real QFF stations onto the paper grid 2400x1200: FP32C and FP64 drop-in;
+    * method='naive' and method='radius' (FP64 brute force) on one 2400x1200 field."""
+    from fastbarnes import hip, realdata
+    from fastbarnes.batched import BarnesBatch
+    from fastbarnes.interpolation import barnes
+    from fastbarnes.interpolationS2 import barnes_S2
+    from oracle import oracle
+    log('rows: convolution, barnes_S2, naive, radius')
+    res = {}
+    c0 = cfgs[0]
+    args0 = (c0['pts'], c0['val'], c0['sigma'], c0['x0'], c0['step'], c0['size'])
+    npf = c0['size'][0] * c0['size'][1]
+    # method='convolution', FP32C batch
+    descs = [dict(d, method='convolution') for d in _descs(cfgs)]
+    batch = BarnesBatch(descs, precision='fp32c')
+    pts = np.concatenate([c['pts'] for c in cfgs])
+    val = np.concatenate([c['val'] for c in cfgs])
+    d_pts, d_val = hip.DeviceArray.from_numpy(pts), hip.DeviceArray.from_numpy(val)
+    d_out = hip.DeviceArray((batch.total_out,), np.float32)
+    s = hip.Stream()
+    if AUTOTUNE:
+        batch.autotune(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr)
+    ms = _time_steps(lambda: batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr, graph=True), s, steps, 2)
+    f0 = d_out.numpy()[:npf].reshape(c0['size'][::-1])
+    t64, ref64 = _best_of(lambda: barnes(*args0, method='convolution', precision='fp64'))
+    res['convolution'] = dict(workload='%d fields 2400x1200, N=3490, n=4, method=convolution' % len(cfgs),
+                              fp32c_batch_grid_pts_per_s=round(batch.total_out / (ms * 1e-3), 1),
+                              fp32c_batch_ms=round(ms, 4), fp32c_vs_fp64=_err_vs(f0, ref64),
+                              fp64_dropin_ms_per_field=round(t64 * 1e3, 3))
+    # barnes_S2 on the real stations
+    rp, rv = realdata.read_csv_array(os.path.join(ROOT, 'tests', 'golden', 'qff', 'PressQFF_202007271200_3490.csv'))
+    x0, step, size = realdata.paper_grid(32.0)
+    t32, s32 = _best_of(lambda: barnes_S2(rp, rv, 1.0, x0, step, size, precision='fp32c'))
+    ts64, s64 = _best_of(lambda: barnes_S2(rp, rv, 1.0, x0, step, size, precision='fp64'))
+    xs, ss, sz = realdata.paper_grid(4.0)
+    tc = time.perf_counter()
+    o_s2 = oracle.barnes_s2_opt(rp, rv, 1.0, xs, ss, sz)
+    tc = time.perf_counter() - tc
+    g_s2 = barnes_S2(rp, rv, 1.0, xs, ss, sz, precision='fp64')
+    res['barnes_S2'] = dict(workload='3490 QFF stations (lon, lat) onto the paper grid 2400x1200, sigma=1, n=4',
+                            fp32c_dropin_ms=round(t32 * 1e3, 3), fp64_dropin_ms=round(ts64 * 1e3, 3),
+                            fp32c_vs_fp64=_err_vs(s32, s64),
+                            cpu_oracle_s_at_300x150=round(tc, 3), fp64_vs_oracle_at_300x150=_err_vs(g_s2, o_s2))
+    # naive and radius: FP64 brute force over all stations (radius: within max_dist only)
+    for m in ('naive', 'radius'):
+        tg, og = _best_of(lambda: barnes(*args0, method=m), reps=2)
+        rows = 12   # CPU oracle on the first rows of a coarser copy of the grid
+        sub = (c0['size'][0] // 10, rows)
+        t0 = time.perf_counter()
+        if m == 'naive':
+            oc = oracle.interpolate_naive(c0['pts'], c0['val'], c0['sigma'], c0['x0'], c0['step'] * 10, sub)
+        else:
+            oc = oracle.interpolate_radius(c0['pts'], c0['val'], c0['sigma'], c0['x0'], c0['step'] * 10, sub)
+        tcpu = time.perf_counter() - t0
+        gs = barnes(c0['pts'], c0['val'], c0['sigma'], c0['x0'], c0['step'] * 10, sub, method=m)
+        res[m] = dict(workload='one 2400x1200 field, N=3490 (drop-in, numpy in and out)',
+                      gpu_ms=round(tg * 1e3, 3), gpu_grid_pts_per_s=round(npf / tg, 1),
+                      cpu_oracle_grid_pts_per_s=round(sub[0] * sub[1] / tcpu, 1), cpu_sample='%dx%d nodes' % sub,
+                      gpu_vs_oracle_on_sample=_err_vs(gs, oc))
+    return res
+
+
 def run_legs(names, args):
     from fastbarnes import workloads
     legs = {}
@@ -248,6 +335,8 @@ def run_legs(names, args):
                                        'fp32c, hipGraph replay')
     if 'real' in names:
         legs['real'] = leg_real(args.leg_steps)
+    if 'rows' in names:
+        legs['rows'] = leg_rows(make_fields(args.fields)[0], args.leg_steps)
     if 'cfg5' in names:
         cfgs = [workloads.config5(seed=500 + i) for i in range(16)]
         legs['cfg5'] = leg_train(cfgs, args.leg_steps, 2)
@@ -263,7 +352,7 @@ def main():
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-extras', action='store_true', help='skip accuracy and FP64-path legs')
-    ap.add_argument('--legs', default='cfg3,cfg4,cfg5,real',
+    ap.add_argument('--legs', default='cfg3,cfg4,cfg5,real,rows',
                     help="comma list of extra workloads reported under 'workloads' (cfg3, cfg4, cfg5, real) "
                          "or 'none'")
     ap.add_argument('--leg-steps', type=int, default=10)
