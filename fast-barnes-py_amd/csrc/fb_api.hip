@@ -1024,7 +1024,6 @@ int fb_plan_autotune(fb_plan *P, const double *d_pts, const double *d_val, float
             while (!cl.second[f0]) ++f0;
             const PassChoice base = model_choice(cl.first, P->fh[f0].T[a]);
             PassChoice best{-1, kBlock, kPassSlide};
-            float best_ms = 0.f;
             std::vector<PassChoice> cands{base};
             auto same = [](const PassChoice &x, const PassChoice &y) {
                 return x.seg == y.seg && x.lpf == y.lpf && x.kind == y.kind;
@@ -1035,23 +1034,35 @@ int fb_plan_autotune(fb_plan *P, const double *d_pts, const double *d_val, float
                 for (int seg : kSegs)
                     if (!same({seg, lpf, kPassSlide}, base)) cands.push_back({seg, lpf, kPassSlide});
             }
+            std::vector<PassChoice> adm;
             for (const PassChoice &cand : cands) {
                 bool ok = true;
                 for (int f = 0; f < P->nfields && ok; ++f)
                     if (cl.second[f]) choice_cost(cl.first, P->fh[f].T[a], cand, &ok);
-                if (!ok) continue;
-                P->seg_choice[{a, cl.first}] = cand;
-                std::vector<PassWork> work;
-                std::vector<PassField> pf;
-                std::vector<PassGroup> groups;
-                if ((rc = build_pass(P, a, work, pf, groups))) return rc;
-                float ms = 0.f;
-                if ((rc = time_pass_subset(P, a, work, pf, groups, cl.second, d_pts, d_val, d_out, s, &ms)))
-                    return rc;
-                if (best.seg < 0 || ms < 0.98f * best_ms) {
-                    best = cand;
-                    best_ms = ms;
+                if (ok) adm.push_back(cand);
+            }
+            // two interleaved rounds over the candidates, each keeping its faster time, so that
+            // clock drift during the sweep favours none of them
+            std::vector<float> tms(adm.size(), 0.f);
+            for (int round = 0; round < 2; ++round)
+                for (size_t c = 0; c < adm.size(); ++c) {
+                    P->seg_choice[{a, cl.first}] = adm[c];
+                    std::vector<PassWork> work;
+                    std::vector<PassField> pf;
+                    std::vector<PassGroup> groups;
+                    if ((rc = build_pass(P, a, work, pf, groups))) return rc;
+                    float ms = 0.f;
+                    if ((rc = time_pass_subset(P, a, work, pf, groups, cl.second, d_pts, d_val, d_out, s, &ms)))
+                        return rc;
+                    tms[c] = round == 0 ? ms : std::min(tms[c], ms);
                 }
+            if (!adm.empty()) {
+                size_t ib = 0, im = 0;   // the base (when admissible) and the fastest candidate
+                for (size_t c = 0; c < adm.size(); ++c) {
+                    if (same(adm[c], base)) ib = c;
+                    if (tms[c] < tms[im]) im = c;
+                }
+                best = same(adm[ib], base) && !(tms[im] < 0.98f * tms[ib]) ? adm[ib] : adm[im];
             }
             if (best.seg < 0) P->seg_choice.erase({a, cl.first});
             else P->seg_choice[{a, cl.first}] = best;
