@@ -157,10 +157,14 @@ __global__ __launch_bounds__(kLocalSortThreads) void k_field_sort(const FieldDev
     for (int i = tid; i < np2; i += kLocalSortThreads)
         sk[i] = i < n ? ((unsigned long long)keys[p0 + i] << 32) | (unsigned long long)(uint32_t)(p0 + i) : ~0ull;
     __syncthreads();
+    // Pair t of a stage is (i, i + j) with i = t with a zero bit inserted at bit log2(j).  A
+    // thread takes pairs t = tid + m*kLocalSortThreads, so for j <= 64 wave w touches only the
+    // 128-element blocks w + 16m: consecutive stages that both have j <= 64 depend on the
+    // wave's own program order only; a barrier follows every stage where this stage or the
+    // next one has j > 64.
     for (int k = 2; k <= np2; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
             for (int t = tid; t < (np2 >> 1); t += kLocalSortThreads) {
-                // the t-th compare-exchange pair (i, i + j) with bit j of i clear
                 const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
                 const int l = i + j;
                 const unsigned long long a = sk[i], b = sk[l];
@@ -170,9 +174,12 @@ __global__ __launch_bounds__(kLocalSortThreads) void k_field_sort(const FieldDev
                     sk[l] = a;
                 }
             }
-            __syncthreads();
+            const int jn = j > 1 ? (j >> 1) : k;   // the next stage's distance
+            if (j > 64 || jn > 64) __syncthreads();
+            else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
     }
+    __syncthreads();
     const uint32_t dst = row_start[fields[f].xline_off];
     for (int i = tid; i < n; i += kLocalSortThreads) {
         const unsigned long long e = sk[i];
