@@ -472,6 +472,38 @@ struct fb_plan {
 
 namespace fb {
 namespace {
+// the pass choice of an FP32C plan for lines of length L along `axis` (autotuned, forced or the
+// cost model's)
+PassChoice pass_choice(const fb_plan *P, int axis, int64_t L, int T)
+{
+    auto it = P->seg_choice.find({axis, L});
+    return it != P->seg_choice.end() ? it->second : model_choice(L, T);
+}
+
+// Whether field f of a 2D FP32C plan keeps its intermediate between the x and y passes blocked
+// (blocked_index): W a multiple of 4 and H of 8, a float4-aligned cell offset, the sliding
+// pass on both axes, y tiles a multiple of 4 columns that kBlkRows float4 rows per thread cover.
+bool field_blocked(const fb_plan *P, int f)
+{
+    const FieldDev &F = P->fh[f];
+    if (F.dim != 2 || F.size[0] % 4 || F.size[1] % 8 || F.cell_off % 4) return false;
+    const PassChoice cx = pass_choice(P, 0, F.size[0], F.T[0]);
+    const PassChoice cy = pass_choice(P, 1, F.size[1], F.T[1]);
+    if (cx.kind != kPassSlide || cy.kind != kPassSlide) return false;
+    int lpl, lines, rpad;
+    size_t lds;
+    const std::string keep = last_error();
+    const int rc = pass_geometry(F.size[1], F.T[1], cy.seg, cy.lpf, lpl, lines, rpad, lds);
+    set_error(keep);
+    if (rc || lines % 4) return false;
+    const int64_t rstep = 2 * cy.lpf / (lines / 4);
+    return rstep >= 1 && rstep * kBlkRows >= F.size[1];
+}
+}  // namespace
+}  // namespace fb
+
+namespace fb {
+namespace {
 
 bool finite_pos(double x) { return std::isfinite(x) && x > 0.0; }
 
@@ -548,8 +580,7 @@ int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<Pa
         const FieldDev &F = P->fh[f];
         const int64_t W = F.size[0], H = F.size[1], D = F.size[2];
         const int64_t L = F.size[axis];
-        auto it = P->seg_choice.find({axis, L});
-        const PassChoice ch = it != P->seg_choice.end() ? it->second : model_choice(L, F.T[axis]);
+        const PassChoice ch = pass_choice(P, axis, L, F.T[axis]);
         const int seg = ch.seg;
         int lpl = 1, lines = 1, rpad = 0, wgs = 0;
         size_t l = 0;
@@ -566,6 +597,7 @@ int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<Pa
         p.rpad = rpad;
         p.seg = seg;
         p.alpha = F.alpha[axis];
+        p.blocked = axis < 2 && field_blocked(P, f) ? (axis == 0 ? kBlockedOut : kBlockedIn) : kRowMajor;
         if (F.pscale[axis] != 1.0)
             scale_groups((double)(2 * F.T[axis] + 1) + 2 * F.alpha[axis], F.num_iter, F.pscale[axis], p);
         else
@@ -593,7 +625,13 @@ int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<Pa
     work.clear();
     groups.clear();
     for (auto &kv : by_seg) {
-        std::vector<PassWork> items = axis == 0 ? kv.second : xcd_order(kv.second);
+        // x passes too when they write a blocked intermediate: the 4 workgroups whose rows share
+        // its lines then run on one XCD, so their partial lines merge in its L2 (measured
+        // slower for row-major x passes of 3D fields)
+        bool any_blocked = false;
+        for (const PassWork &w : kv.second)
+            if (pf[w.field].blocked != kRowMajor) { any_blocked = true; break; }
+        std::vector<PassWork> items = axis == 0 && !any_blocked ? kv.second : xcd_order(kv.second);
         groups.push_back({(kv.first / 1024) % 1024, kv.first % 1024, kv.first / (1024 * 1024), (int)work.size(),
                           (int)items.size(), lds_of[kv.first]});
         work.insert(work.end(), items.begin(), items.end());
@@ -951,7 +989,7 @@ int fb_plan_backward(fb_plan *P, const double *d_pts, const double *d_val, const
                            a == 0 ? const_cast<float *>(d_wsave) : nullptr, 0, s, a == 0 ? d_gout : nullptr)))
             return rc;
     return launch_adj_gather(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints, d_pts, d_val, P->d_offsets,
-                             P->d_v, P->d_w, d_gval, d_gpts, s);
+                             P->d_v, P->d_w, P->dim >= 2 ? P->d_pass[1] : nullptr, d_gval, d_gpts, s);
 }
 
 // Device time (ms, mean of 5 after a warm-up) of the work items of the fields selected by
@@ -1125,6 +1163,14 @@ int fb_plan_pass_groups(fb_plan *P, int axis, int32_t *seg, int32_t *threads, in
         if (items) items[i] = g[i].count;
         if (kind) kind[i] = g[i].kind;
     }
+    return FB_OK;
+}
+
+int fb_plan_field_layout(fb_plan *P, int field, int *blocked)
+{
+    if (!P || !blocked) return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (field < 0 || field >= P->nfields) return fail(FB_ERR_INVALID_ARGUMENT, "field out of range");
+    *blocked = P->precision == FB_PREC_FP32C && field_blocked(P, field) ? 1 : 0;
     return FB_OK;
 }
 

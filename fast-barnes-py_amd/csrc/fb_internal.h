@@ -76,7 +76,26 @@ struct PassField {
     int32_t seg;           // elements per lane (the kernel instantiation the pass runs)
     double sgscale;
     double slast;
+    // Layout of the intermediate v, w fields between the x and y passes of a 2D field: row-major
+    // (element (y, x) at y*W + x) unless the x pass writes them blocked (blocked_index,
+    // kBlockedOut) and the y pass reads them so (kBlockedIn).  Every other pass input and
+    // output is row-major.
+    int32_t blocked;
+    int32_t pad_;
 };
+
+enum BlockedRole { kRowMajor = 0, kBlockedOut = 1, kBlockedIn = 2 };
+
+// float4 rows per thread of a blocked y-pass tile load (all in flight at once)
+constexpr int kBlkRows = 5;
+
+// Blocked intermediate of a 2D field (W a multiple of 4, H of 8): blocks of 8 rows x 4
+// columns, one 128-byte line each, block (y/8, x/4) at line (y/8)*(W/4) + x/4, element
+// (y%8, x%4) at float (y%8)*4 + x%4 of it.  A y-pass tile of 4 columns reads whole lines.
+__host__ __device__ inline int64_t blocked_index(int64_t y, int64_t x, int64_t W)
+{
+    return (((y >> 3) * (W >> 2) + (x >> 2)) << 5) + ((y & 7) << 2) + (x & 3);
+}
 
 // Contribution record of the sorted injection list.
 struct Contrib {

@@ -658,6 +658,80 @@ __device__ __forceinline__ void tile_copy(float *__restrict__ g, int64_t ls, int
     }
 }
 
+// x-pass store of nl rows (y0 ..) of a 2D field into the blocked intermediate: consecutive
+// threads take the rows of one 4-column group, so a 2-row tile writes 32-byte pieces.
+__device__ __forceinline__ void blocked_store_rows(float *__restrict__ g, int64_t W, int64_t y0, int nl, int L,
+                                                   const float *__restrict__ lds, int lstride)
+{
+    const int n = nl * (L >> 2);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int l = i % nl, q = i / nl;
+        const float *d = lds + l * lstride + 4 * q;
+        *reinterpret_cast<float4 *>(g + blocked_index(y0 + l, 4 * q, W)) = make_float4(d[0], d[1], d[2], d[3]);
+    }
+}
+
+// y-pass load of the columns x0 .. x0+nl-1 (nl a multiple of 4) of both intermediate fields
+// from the blocked layout into LDS lines (line l, element j at lds[l*lstride + j]).  Thread t
+// takes column group t % (nl/4) and rows t / (nl/4) + k*rstep, k < kBlkRows (the host checks
+// that they cover the column); all loads are issued before the first LDS write.
+__device__ __forceinline__ void blocked_load_cols(const float *__restrict__ gv, const float *__restrict__ gw,
+                                                  int64_t W, int64_t x0, int nl, int L, float *__restrict__ dv,
+                                                  float *__restrict__ dw, int lstride)
+{
+    const int nq = nl >> 2;
+    const int rstep = blockDim.x / nq;
+    const int q = threadIdx.x % nq, r0 = threadIdx.x / nq;
+    if (r0 >= rstep) return;
+    float4 rv[kBlkRows], rw[kBlkRows];
+#pragma unroll
+    for (int k = 0; k < kBlkRows; ++k) {
+        const int r = r0 + k * rstep;
+        if (r < L) {
+            const int64_t o = blocked_index(r, x0 + 4 * q, W);
+            rv[k] = *reinterpret_cast<const float4 *>(gv + o);
+            rw[k] = *reinterpret_cast<const float4 *>(gw + o);
+        }
+    }
+    float *a = dv + (4 * q) * lstride + r0, *b = dw + (4 * q) * lstride + r0;
+#pragma unroll
+    for (int k = 0; k < kBlkRows; ++k)
+        if (r0 + k * rstep < L) {
+            const int e = k * rstep;
+            a[e] = rv[k].x;
+            a[lstride + e] = rv[k].y;
+            a[2 * lstride + e] = rv[k].z;
+            a[3 * lstride + e] = rv[k].w;
+            b[e] = rw[k].x;
+            b[lstride + e] = rw[k].y;
+            b[2 * lstride + e] = rw[k].z;
+            b[3 * lstride + e] = rw[k].w;
+        }
+}
+
+// The inverse of blocked_load_cols: a y pass that is not the last one (the adjoint's) writes
+// its tile back in place, in the blocked layout it was read in.
+__device__ __forceinline__ void blocked_store_cols(float *__restrict__ gv, float *__restrict__ gw, int64_t W,
+                                                   int64_t x0, int nl, int L, const float *__restrict__ dv,
+                                                   const float *__restrict__ dw, int lstride)
+{
+    const int nq = nl >> 2;
+    const int rstep = blockDim.x / nq;
+    const int q = threadIdx.x % nq, r0 = threadIdx.x / nq;
+    if (r0 >= rstep) return;
+    const float *a = dv + (4 * q) * lstride + r0, *b = dw + (4 * q) * lstride + r0;
+#pragma unroll
+    for (int k = 0; k < kBlkRows; ++k) {
+        const int r = r0 + k * rstep;
+        if (r < L) {
+            const int e = k * rstep;
+            const int64_t o = blocked_index(r, x0 + 4 * q, W);
+            *reinterpret_cast<float4 *>(gv + o) = make_float4(a[e], a[lstride + e], a[2 * lstride + e], a[3 * lstride + e]);
+            *reinterpret_cast<float4 *>(gw + o) = make_float4(b[e], b[lstride + e], b[2 * lstride + e], b[3 * lstride + e]);
+        }
+    }
+}
+
 // Load of the adjoint's first pass (rows, element stride 1): from the forward output f, the
 // saved normalised weights W and dL/df = g of a row tile it forms a = g / W and
 // b = -a (f - offset) (the two fields the adjoint filters; zero where f or g is NaN) straight
@@ -847,6 +921,9 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
         if (adj_g)
             adjoint_load(out + F.out_off + wk.base, wsave + F.out_off + wk.base, adj_g + F.out_off + wk.base, off,
                          P.line_stride, nl, L, sv + pad, sw + pad, lstride);
+        else if (P.blocked == kBlockedIn)
+            blocked_load_cols(gv + F.cell_off, gw + F.cell_off, F.size[0], wk.base, nl, L, sv + pad, sw + pad,
+                              lstride);
         else {
             tile_copy<true>(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
             tile_copy<true>(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
@@ -908,6 +985,12 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
         tile_copy<false>(out + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
         // training forward: keep the normalised weight field for the adjoint
         if (wsave) tile_copy<false>(wsave + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
+    } else if (P.blocked == kBlockedOut) {
+        const int64_t y0 = wk.base / F.size[0];
+        blocked_store_rows(gv + F.cell_off, F.size[0], y0, nl, L, sv + pad, lstride);
+        blocked_store_rows(gw + F.cell_off, F.size[0], y0, nl, L, sw + pad, lstride);
+    } else if (P.blocked == kBlockedIn) {
+        blocked_store_cols(gv + F.cell_off, gw + F.cell_off, F.size[0], wk.base, nl, L, sv + pad, sw + pad, lstride);
     } else {
         tile_copy<false>(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
         tile_copy<false>(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
@@ -1322,6 +1405,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_gather(const FieldDev *__restric
                                                        const double *__restrict__ val,
                                                        const double *__restrict__ offsets,
                                                        const float *__restrict__ A, const float *__restrict__ Bb,
+                                                       const PassField *__restrict__ pass_y,
                                                        double *__restrict__ gval, double *__restrict__ gpts)
 {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -1329,6 +1413,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_gather(const FieldDev *__restric
     const int f = find_seg(pt_prefix, nfields, i);
     const FieldDev &F = fields[f];
     const int dim = F.dim;
+    // A and B of a 2D field whose y pass ran on the blocked intermediate are blocked
+    const bool blocked = pass_y && pass_y[f].blocked == kBlockedIn;
     PointPos pp;
     const bool inside = point_pos(F, pts + i * dim, pp);
     double gv = 0.0, gp[3] = {0.0, 0.0, 0.0};
@@ -1338,7 +1424,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_gather(const FieldDev *__restric
         const int corners = 1 << dim;
         for (int cb = 0; cb < corners; ++cb) {
             const int bx = cb & 1, by = (cb >> 1) & 1, bz = (cb >> 2) & 1;
-            const int64_t cell = F.cell_off + ((pp.i[2] + bz) * H + (pp.i[1] + by)) * W + (pp.i[0] + bx);
+            const int64_t cell = F.cell_off + (blocked ? blocked_index(pp.i[1] + by, pp.i[0] + bx, W)
+                                                       : ((pp.i[2] + bz) * H + (pp.i[1] + by)) * W + (pp.i[0] + bx));
             const double a = (double)A[cell], b = (double)Bb[cell];
             const double fx = bx ? pp.w[0] : 1.0 - pp.w[0];
             const double fy = dim >= 2 ? (by ? pp.w[1] : 1.0 - pp.w[1]) : 1.0;
@@ -1546,11 +1633,11 @@ int launch_fast_pass(int kind, bool inject, int seg, const FieldDev *d_fields, c
 
 int launch_adj_gather(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
                       const double *d_pts, const double *d_val, const double *d_offsets, const float *d_A,
-                      const float *d_Bb, double *d_gval, double *d_gpts, hipStream_t s)
+                      const float *d_Bb, const PassField *d_pass_y, double *d_gval, double *d_gpts, hipStream_t s)
 {
     if (npoints <= 0) return FB_OK;
     hipLaunchKernelGGL(k_adj_gather, dim3(nblocks(npoints)), dim3(kBlock), 0, s, d_fields, d_pt_prefix, nfields,
-                       npoints, d_pts, d_val, d_offsets, d_A, d_Bb, d_gval, d_gpts);
+                       npoints, d_pts, d_val, d_offsets, d_A, d_Bb, d_pass_y, d_gval, d_gpts);
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }

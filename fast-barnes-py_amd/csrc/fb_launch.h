@@ -66,7 +66,8 @@ int launch_fast_pass(int kind, bool inject, int seg, const FieldDev *d_fields, c
 // fb_kernels.hip -- adjoint of the FP32C batch
 int launch_adj_gather(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
                       const double *d_pts, const double *d_val, const double *d_offsets, const float *d_A,
-                      const float *d_Bb, double *d_gval, double *d_gpts, hipStream_t stream);
+                      const float *d_Bb, const PassField *d_pass_y, double *d_gval, double *d_gpts,
+                      hipStream_t stream);
 
 // fb_s2.hip -- spherical variant (proj = center_lon, n, n_inv, F, rho0)
 int launch_lambert_points(const double *d_pts, double *d_out, int64_t n, const double *proj, hipStream_t stream);

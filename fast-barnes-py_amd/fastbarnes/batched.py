@@ -141,6 +141,13 @@ class BarnesBatch:
         check(lib().fb_plan_pass_groups(self._plan, axis, seg, thr, items, kind, 32, ctypes.byref(n)))
         return [(('slide', 'prefix')[kind[i]], seg[i], thr[i], items[i]) for i in range(min(n.value, 32))]
 
+    def intermediate_blocked(self, field):
+        """True when field `field` keeps its intermediate between the x and y passes in blocks
+        of 8 rows x 4 columns (FP32C 2D fields that allow it; same results either way)."""
+        b = ctypes.c_int()
+        check(lib().fb_plan_field_layout(self._plan, int(field), ctypes.byref(b)))
+        return bool(b.value)
+
     def set_timing(self, enable):
         check(lib().fb_plan_set_timing(self._plan, int(bool(enable))))
 

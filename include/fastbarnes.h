@@ -196,6 +196,13 @@ int fb_plan_choose_pass(fb_plan *plan, int axis, int64_t len, int kind, int seg,
 int fb_plan_pass_groups(fb_plan *plan, int axis, int32_t *seg, int32_t *threads, int32_t *items, int32_t *kind,
                         int max_groups, int *ngroups);
 
+/* Layout of field `field`'s intermediate v, w fields between the x and y passes of an FP32C
+ * plan: *blocked = 1 when they are kept in blocks of 8 rows x 4 columns (one 128-byte line
+ * each, so that a y-pass tile of 4 columns reads whole lines), 0 when row-major.  2D fields
+ * with W a multiple of 4 and H of 8 whose passes allow it are blocked; the results are the
+ * same bits either way.  No reference counterpart (the reference has no GPU layout). */
+int fb_plan_field_layout(fb_plan *plan, int field, int *blocked);
+
 /* Per-stage device time (ms) of the last fb_plan_execute run with timing enabled,
  * measured with HIP events recorded on the execution stream.  names receives a
  * comma-separated stage list.  enable != 0 turns event recording on. */

@@ -214,6 +214,59 @@ def test_prefix_pass_matches_fp64(seg):
         assert float(np.max(np.abs(r[ok] - slide[ok]) / (np.abs(slide[ok]) + 1.0))) < 1e-5
 
 
+def test_blocked_intermediate_matches_row_major():
+    """2D fields with W a multiple of 4 and H of 8 keep the intermediate between the x and y
+    passes in blocks of 8 rows x 4 columns when their y tiles are 4k columns wide (256-thread
+    y pass, 4-column tiles); with 128-thread y workgroups (2-column tiles) it stays row-major.  Same segment
+    length, so the same arithmetic: forward results, graph replays, the training forward's
+    saved weights and the adjoint's gradients must be the same bits.  The 600x300 field (H not
+    a multiple of 8) stays row-major in both plans."""
+    from fastbarnes import hip, workloads
+    from fastbarnes.batched import BarnesBatch
+    group = [workloads.config2(seed=150), workloads.config3_field(0), workloads.config2(seed=151)]
+    group += [c for c in (workloads.config3_field(i) for i in range(1, 12)) if c['size'][1] in (300, 600)][:2]
+    shapes = [tuple(c['size']) for c in group]
+    assert (1200, 600) in shapes and (600, 300) in shapes, shapes
+    pts = np.concatenate([c['pts'] for c in group])
+    val = np.concatenate([c['val'] for c in group])
+    d_pts, d_val = hip.DeviceArray.from_numpy(pts), hip.DeviceArray.from_numpy(val)
+    s = hip.Stream()
+    res = {}
+    for threads in (256, 128):
+        batch = BarnesBatch([_desc(c) for c in group], precision='fp32c')
+        for H in sorted({c['size'][1] for c in group}):
+            batch.choose_pass(1, H, 'slide', 39, threads)
+        blocked = [batch.intermediate_blocked(f) for f in range(len(group))]
+        if threads == 256:
+            # the 2400x1200 fields (4-column y tiles) are blocked; a field whose y tiles hold a
+            # number of columns that is not a multiple of 4 (6 lines of a large kernel fit the
+            # LDS) stays row-major, like the 600x300 field
+            assert blocked[0] and blocked[2] and not blocked[4], blocked
+        else:
+            assert not any(blocked)
+        d_out = hip.DeviceArray((batch.total_out,), np.float32)
+        batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr)
+        s.synchronize()
+        out = d_out.numpy()
+        batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr, graph=True)
+        s.synchronize()
+        _bitwise(d_out.numpy(), out)
+        d_ws = hip.DeviceArray((batch.total_out,), np.float32)
+        batch.forward(d_pts.ptr, d_val.ptr, d_out.ptr, d_ws.ptr, s.ptr)
+        s.synchronize()
+        fwd = d_out.numpy()
+        g = np.nan_to_num(fwd - np.float32(0.5), nan=0.0).astype(np.float32)
+        d_g = hip.DeviceArray.from_numpy(g)
+        d_gval = hip.DeviceArray((len(val),), np.float64)
+        d_gpts = hip.DeviceArray(pts.shape, np.float64)
+        batch.backward(d_pts.ptr, d_val.ptr, d_out.ptr, d_ws.ptr, d_g.ptr, d_gval.ptr, d_gpts.ptr, s.ptr)
+        s.synchronize()
+        res[threads] = (out, fwd, d_ws.numpy(), d_gval.numpy(), d_gpts.numpy())
+    assert np.mean(~np.isnan(res[256][0])) > 0.2
+    for a, b in zip(res[256], res[128]):
+        _bitwise(a, b)
+
+
 def test_choose_pass_rejects_bad_choices():
     from fastbarnes.batched import BarnesBatch
     c = _fields(3)[0]
