@@ -663,11 +663,24 @@ __device__ __forceinline__ void tile_copy(float *__restrict__ g, int64_t ls, int
 __device__ __forceinline__ void blocked_store_rows(float *__restrict__ g, int64_t W, int64_t y0, int nl, int L,
                                                    const float *__restrict__ lds, int lstride)
 {
-    const int n = nl * (L >> 2);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const int l = i % nl, q = i / nl;
-        const float *d = lds + l * lstride + 4 * q;
-        *reinterpret_cast<float4 *>(g + blocked_index(y0 + l, 4 * q, W)) = make_float4(d[0], d[1], d[2], d[3]);
+    const int nq = L >> 2;
+    const int nthr = blockDim.x;
+    if (nthr % nl == 0) {
+        // thread t keeps row t % nl and steps through the 4-column groups t / nl + k*(nthr/nl):
+        // consecutive groups of a row are 32 floats apart in the blocked layout
+        const int l = threadIdx.x % nl;
+        const int qstep = nthr / nl;
+        const float *d = lds + l * lstride;
+        float *gp = g + blocked_index(y0 + l, 0, W);
+        for (int q = threadIdx.x / nl; q < nq; q += qstep)
+            *reinterpret_cast<float4 *>(gp + 32 * (int64_t)q) =
+                make_float4(d[4 * q], d[4 * q + 1], d[4 * q + 2], d[4 * q + 3]);
+    } else {
+        for (int i = threadIdx.x; i < nl * nq; i += nthr) {
+            const int l = i % nl, q = i / nl;
+            const float *d = lds + l * lstride + 4 * q;
+            *reinterpret_cast<float4 *>(g + blocked_index(y0 + l, 4 * q, W)) = make_float4(d[0], d[1], d[2], d[3]);
+        }
     }
 }
 
