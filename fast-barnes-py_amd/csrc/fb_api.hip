@@ -430,6 +430,7 @@ struct fb_plan {
     std::vector<PassGroup> groups[3];          // launches of each axis pass, one per segment length
     std::map<std::pair<int, int64_t>, PassChoice> seg_choice;   // (axis, line length) -> autotuned choice
     const uint32_t *sorted_vals = nullptr;      // sorted point indices of the last run
+    int allow_blocked = 1;                      // fb_plan_set_blocked
 
     int timing = 0;
     std::vector<hipEvent_t> events;
@@ -481,12 +482,13 @@ PassChoice pass_choice(const fb_plan *P, int axis, int64_t L, int T)
 }
 
 // Whether field f of a 2D FP32C plan keeps its intermediate between the x and y passes blocked
-// (blocked_index): W a multiple of 4 and H of 8, a float4-aligned cell offset, the sliding
-// pass on both axes, y tiles a multiple of 4 columns that kBlkRows float4 rows per thread cover.
+// (blocked_index): W a multiple of 4 (plan_create then gives the field room for H rounded up
+// to 8 rows), a float4-aligned cell offset, the sliding pass on both axes, y tiles a multiple
+// of 4 columns that kBlkRows float4 rows per thread cover.
 bool field_blocked(const fb_plan *P, int f)
 {
     const FieldDev &F = P->fh[f];
-    if (F.dim != 2 || F.size[0] % 4 || F.size[1] % 8 || F.cell_off % 4) return false;
+    if (!P->allow_blocked || F.dim != 2 || F.size[0] % 4 || F.cell_off % 4) return false;
     const PassChoice cx = pass_choice(P, 0, F.size[0], F.T[0]);
     const PassChoice cy = pass_choice(P, 1, F.size[1], F.T[1]);
     if (cx.kind != kPassSlide || cy.kind != kPassSlide) return false;
@@ -707,7 +709,10 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
         }
         P->npoints += d.n;
         P->nxlines += H * D;
-        P->ncells += (W * H * D + 63) / 64 * 64;   // 256-byte aligned field starts (float4 tiles)
+        // 256-byte aligned field starts (float4 tiles); an FP32C 2D field that may keep its
+        // intermediate blocked (8-row blocks) gets room for H rounded up to 8 rows
+        const int64_t Hc = fast && P->dim == 2 && W % 4 == 0 ? (H + 7) / 8 * 8 : H;
+        P->ncells += (W * Hc * D + 63) / 64 * 64;
         P->total_cells += W * H * D;
         P->total_out = std::max(P->total_out, d.out_off + W * H * D);
     }
@@ -1163,6 +1168,18 @@ int fb_plan_pass_groups(fb_plan *P, int axis, int32_t *seg, int32_t *threads, in
         if (items) items[i] = g[i].count;
         if (kind) kind[i] = g[i].kind;
     }
+    return FB_OK;
+}
+
+int fb_plan_set_blocked(fb_plan *P, int enable)
+{
+    if (!P) return fail(FB_ERR_INVALID_ARGUMENT, "plan is NULL");
+    P->allow_blocked = enable ? 1 : 0;
+    if (P->precision != FB_PREC_FP32C) return FB_OK;
+    int rc = build_passes(P);
+    if (rc) return rc;
+    if (P->gexec) { (void)hipGraphExecDestroy(P->gexec); P->gexec = nullptr; }
+    if (P->graph) { (void)hipGraphDestroy(P->graph); P->graph = nullptr; }
     return FB_OK;
 }
 

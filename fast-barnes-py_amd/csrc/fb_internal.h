@@ -89,9 +89,10 @@ enum BlockedRole { kRowMajor = 0, kBlockedOut = 1, kBlockedIn = 2 };
 // float4 rows per thread of a blocked y-pass tile load (all in flight at once)
 constexpr int kBlkRows = 5;
 
-// Blocked intermediate of a 2D field (W a multiple of 4, H of 8): blocks of 8 rows x 4
-// columns, one 128-byte line each, block (y/8, x/4) at line (y/8)*(W/4) + x/4, element
-// (y%8, x%4) at float (y%8)*4 + x%4 of it.  A y-pass tile of 4 columns reads whole lines.
+// Blocked intermediate of a 2D field (W a multiple of 4; the field's cell space holds H
+// rounded up to 8 rows): blocks of 8 rows x 4 columns, one 128-byte line each, block
+// (y/8, x/4) at line (y/8)*(W/4) + x/4, element (y%8, x%4) at float (y%8)*4 + x%4 of it.  A
+// y-pass tile of 4 columns reads whole lines.
 __host__ __device__ inline int64_t blocked_index(int64_t y, int64_t x, int64_t W)
 {
     return (((y >> 3) * (W >> 2) + (x >> 2)) << 5) + ((y & 7) << 2) + (x & 3);

@@ -88,6 +88,7 @@ SIGNATURES = {
                                            ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_int]),
     'fb_plan_info': (ctypes.c_int, [_vp, _i64p, _i64p]),
     'fb_plan_field_layout': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    'fb_plan_set_blocked': (ctypes.c_int, [_vp, ctypes.c_int]),
     'fb_plan_destroy': (ctypes.c_int, [_vp]),
 }
 

@@ -148,6 +148,10 @@ class BarnesBatch:
         check(lib().fb_plan_field_layout(self._plan, int(field), ctypes.byref(b)))
         return bool(b.value)
 
+    def set_blocked(self, enable):
+        """Allows (default) or forbids the blocked intermediate; the same results either way."""
+        check(lib().fb_plan_set_blocked(self._plan, int(bool(enable))))
+
     def set_timing(self, enable):
         check(lib().fb_plan_set_timing(self._plan, int(bool(enable))))
 

@@ -203,6 +203,11 @@ int fb_plan_pass_groups(fb_plan *plan, int axis, int32_t *seg, int32_t *threads,
  * same bits either way.  No reference counterpart (the reference has no GPU layout). */
 int fb_plan_field_layout(fb_plan *plan, int field, int *blocked);
 
+/* Allows (enable = 1, the default) or forbids (0) the blocked intermediate for the plan's
+ * fields and rebuilds its passes; for tests and measurements (the results are the same bits
+ * either way).  No reference counterpart. */
+int fb_plan_set_blocked(fb_plan *plan, int enable);
+
 /* Per-stage device time (ms) of the last fb_plan_execute run with timing enabled,
  * measured with HIP events recorded on the execution stream.  names receives a
  * comma-separated stage list.  enable != 0 turns event recording on. */

@@ -215,33 +215,40 @@ def test_prefix_pass_matches_fp64(seg):
 
 
 def test_blocked_intermediate_matches_row_major():
-    """2D fields with W a multiple of 4 and H of 8 keep the intermediate between the x and y
-    passes in blocks of 8 rows x 4 columns when their y tiles are 4k columns wide (256-thread
-    y pass, 4-column tiles); with 128-thread y workgroups (2-column tiles) it stays row-major.  Same segment
-    length, so the same arithmetic: forward results, graph replays, the training forward's
-    saved weights and the adjoint's gradients must be the same bits.  The 600x300 field (H not
-    a multiple of 8) stays row-major in both plans."""
+    """2D fields with W a multiple of 4 keep the intermediate between the x and y passes in
+    blocks of 8 rows x 4 columns when their y tiles are 4k columns wide; set_blocked(False)
+    keeps every field row-major with the same pass choices, so the same arithmetic: forward
+    results, graph replays, the training forward's saved weights and the adjoint's gradients
+    must be the same bits."""
     from fastbarnes import hip, workloads
     from fastbarnes.batched import BarnesBatch
     group = [workloads.config2(seed=150), workloads.config3_field(0), workloads.config2(seed=151)]
     group += [c for c in (workloads.config3_field(i) for i in range(1, 12)) if c['size'][1] in (300, 600)][:2]
     shapes = [tuple(c['size']) for c in group]
     assert (1200, 600) in shapes and (600, 300) in shapes, shapes
+    # H not a multiple of 8 (the cell space is padded to whole 8-row blocks), and 8-column y
+    # tiles (two float4 column groups per row: a small kernel fits 8 lines of 600 in LDS)
+    for W, H, sigma, seed in ((2400, 1196, 1.0, 152), (1200, 600, 0.6, 153), (602, 300, 1.0, 154)):
+        p = workloads.stations_2d(2500, seed)
+        group.append(dict(pts=p, val=workloads.values_sinusoids(p, seed), sigma=sigma, x0=np.array([-26.0, 34.0]),
+                          step=75.0 / W, size=(W, H), num_iter=4, max_dist=3.5))
     pts = np.concatenate([c['pts'] for c in group])
     val = np.concatenate([c['val'] for c in group])
     d_pts, d_val = hip.DeviceArray.from_numpy(pts), hip.DeviceArray.from_numpy(val)
     s = hip.Stream()
     res = {}
-    for threads in (256, 128):
+    for allow in (True, False):
         batch = BarnesBatch([_desc(c) for c in group], precision='fp32c')
         for H in sorted({c['size'][1] for c in group}):
-            batch.choose_pass(1, H, 'slide', 39, threads)
+            batch.choose_pass(1, H, 'slide', 39, 256)
+        batch.set_blocked(allow)
         blocked = [batch.intermediate_blocked(f) for f in range(len(group))]
-        if threads == 256:
-            # the 2400x1200 fields (4-column y tiles) are blocked; a field whose y tiles hold a
-            # number of columns that is not a multiple of 4 (6 lines of a large kernel fit the
-            # LDS) stays row-major, like the 600x300 field
-            assert blocked[0] and blocked[2] and not blocked[4], blocked
+        if allow:
+            # the 2400x1200 and 2400x1196 fields (4-column y tiles) and the small-kernel 1200x600
+            # field (8-column tiles) are blocked; a field whose y tiles hold a number of columns
+            # that is not a multiple of 4 (7 lines of a large kernel fit the LDS) stays
+            # row-major, as does the 602-wide field (W not a multiple of 4)
+            assert blocked[0] and blocked[2] and blocked[5] and blocked[6] and not blocked[7], blocked
         else:
             assert not any(blocked)
         d_out = hip.DeviceArray((batch.total_out,), np.float32)
@@ -261,9 +268,9 @@ def test_blocked_intermediate_matches_row_major():
         d_gpts = hip.DeviceArray(pts.shape, np.float64)
         batch.backward(d_pts.ptr, d_val.ptr, d_out.ptr, d_ws.ptr, d_g.ptr, d_gval.ptr, d_gpts.ptr, s.ptr)
         s.synchronize()
-        res[threads] = (out, fwd, d_ws.numpy(), d_gval.numpy(), d_gpts.numpy())
-    assert np.mean(~np.isnan(res[256][0])) > 0.2
-    for a, b in zip(res[256], res[128]):
+        res[allow] = (out, fwd, d_ws.numpy(), d_gval.numpy(), d_gpts.numpy())
+    assert np.mean(~np.isnan(res[True][0])) > 0.2
+    for a, b in zip(res[True], res[False]):
         _bitwise(a, b)
 
 
