@@ -322,9 +322,14 @@ __global__ __launch_bounds__(kBlock) void k_exact_inject(
 
 // _accumulate_tail_array (interpolation.py:484-533) and _accumulate_array (:729-772)
 // on one strided line, ping-ponging between the grid line G and a scratch line S.
-__device__ void exact_line(double *__restrict__ G, double *__restrict__ S, int64_t L, int64_t st, int64_t rect_len,
+constexpr int kExactBlk = 32;  // interior steps of exact_line whose loads are issued together
+
+// UNIT: element stride 1 (rows), known at compile time so that adjacent loads merge
+template <bool UNIT>
+__device__ void exact_line(double *__restrict__ G, double *__restrict__ S, int64_t L, int64_t st_, int64_t rect_len,
                            int num_iter, int tail, double alpha)
 {
+    const int64_t st = UNIT ? 1 : st_;
     const int64_t h0 = (rect_len - 1) / 2;
     const int64_t h0_1 = h0 + 1;
     const int64_t h1 = rect_len - h0;
@@ -341,7 +346,25 @@ __device__ void exact_line(double *__restrict__ G, double *__restrict__ S, int64
                 accu += IN(k + h0);
                 H_(k) = accu + alpha * IN(k + h0_1);
             }
-            for (k = h1; k < L - h0_1; ++k) {
+            // the interior in blocks of kExactBlk steps: the block's input terms are loaded
+            // first (they do not depend on accu; `in` and `h` are different arrays), then the
+            // steps run in the reference's order, so the memory latency is paid once per block
+            for (k = h1; k + kExactBlk <= L - h0_1; k += kExactBlk) {
+                double a[kExactBlk], b[kExactBlk + 1];
+#pragma unroll
+                for (int j = 0; j < kExactBlk; ++j) {
+                    a[j] = IN(k + j + h0);
+                    b[j] = IN(k + j - h1);
+                }
+                b[kExactBlk] = IN(k + kExactBlk + h0);   // IN(k + j + h0_1) is a[j + 1]
+#pragma unroll
+                for (int j = 0; j < kExactBlk; ++j) {
+                    const double nxt = j + 1 < kExactBlk ? a[j + 1] : b[kExactBlk];
+                    accu += (a[j] - b[j]);
+                    H_(k + j) = accu + alpha * (b[j] + nxt);
+                }
+            }
+            for (; k < L - h0_1; ++k) {
                 accu += (IN(k + h0) - IN(k - h1));
                 H_(k) = accu + alpha * (IN(k - h1) + IN(k + h0_1));
             }
@@ -358,7 +381,20 @@ __device__ void exact_line(double *__restrict__ G, double *__restrict__ S, int64
                 accu += IN(k + h0);
                 H_(k) = accu;
             }
-            for (k = h1; k < L - h0; ++k) {
+            for (k = h1; k + kExactBlk <= L - h0; k += kExactBlk) {
+                double a[kExactBlk], b[kExactBlk];
+#pragma unroll
+                for (int j = 0; j < kExactBlk; ++j) {
+                    a[j] = IN(k + j + h0);
+                    b[j] = IN(k + j - h1);
+                }
+#pragma unroll
+                for (int j = 0; j < kExactBlk; ++j) {
+                    accu += (a[j] - b[j]);
+                    H_(k + j) = accu;
+                }
+            }
+            for (; k < L - h0; ++k) {
                 accu += (IN(k + h0) - IN(k - h1));
                 H_(k) = accu;
             }
@@ -400,7 +436,8 @@ __global__ __launch_bounds__(kBlock) void k_exact_lines(const FieldDev *__restri
     const int64_t rect = rect_override > 0 ? rect_override : 2 * (int64_t)F.T[axis] + 1;
     double *G = (which ? wg : vg) + F.cell_off + base;
     double *S = (which ? ws : vs) + F.cell_off + base;
-    exact_line(G, S, L, st, rect, F.num_iter, F.tail, F.alpha[axis]);
+    if (st == 1) exact_line<true>(G, S, L, st, rect, F.num_iter, F.tail, F.alpha[axis]);
+    else exact_line<false>(G, S, L, st, rect, F.num_iter, F.tail, F.alpha[axis]);
 }
 
 // wg[wg < thr] = NaN, then (vg/wg + offset).astype(float32): interpolation.py:428-430, :367
