@@ -173,23 +173,33 @@ __global__ __launch_bounds__(kBlock) void k_point_keys(const FieldDev *__restric
                                                        const int64_t *__restrict__ pt_prefix, int nfields,
                                                        int64_t npoints, const double *__restrict__ pts,
                                                        uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                                                       uint32_t *__restrict__ row_count)
+                                                       uint32_t *__restrict__ row_count,
+                                                       uint32_t *__restrict__ slot, uint32_t *__restrict__ rowg)
 {
-    // row_count == nullptr: the rows' ranges are taken from the sorted keys instead
+    // row_count (may be null): per-row point counts (integer atomics).  slot/rowg (row buckets): the point's
+    // position among its row's points as the atomics handed them out, and its global row
+    // (kInvalidKey outside the grid); vals (point indices) is then not written.
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= npoints) return;
     const int f = find_seg(pt_prefix, nfields, i);
     const FieldDev &F = fields[f];
     PointPos pp;
     const bool inside = point_pos(F, pts + i * F.dim, pp);
-    vals[i] = (uint32_t)i;
+    if (!slot) vals[i] = (uint32_t)i;
     if (!inside) {
         keys[i] = kInvalidKey;
+        if (rowg) rowg[i] = kInvalidKey;
         return;
     }
     const int64_t row = pp.i[2] * F.size[1] + pp.i[1];
     keys[i] = (uint32_t)(F.cell_off + row * F.size[0] + pp.i[0]);
-    if (row_count) atomicAdd(&row_count[F.xline_off + row], 1u);
+    if (row_count) {
+        const uint32_t sl = atomicAdd(&row_count[F.xline_off + row], 1u);
+        if (slot) {
+            slot[i] = sl;
+            rowg[i] = (uint32_t)(F.xline_off + row);
+        }
+    }
 }
 
 // Global x-line (base row) of a sorted key inside the grid.
@@ -1522,11 +1532,12 @@ int launch_offsets(const FieldDev *d_fields, int nfields, int64_t max_n, const d
 }
 
 int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
-                      const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_row_count, hipStream_t s)
+                      const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_row_count,
+                      uint32_t *d_slot, uint32_t *d_rowg, hipStream_t s)
 {
     if (npoints <= 0) return FB_OK;
     hipLaunchKernelGGL(k_point_keys, dim3(nblocks(npoints)), dim3(kBlock), 0, s, d_fields, d_pt_prefix, nfields,
-                       npoints, d_pts, d_keys, d_vals, d_row_count);
+                       npoints, d_pts, d_keys, d_vals, d_row_count, d_slot, d_rowg);
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }

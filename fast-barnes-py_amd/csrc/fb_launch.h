@@ -25,11 +25,21 @@ int launch_field_sort(const FieldDev *d_fields, const int64_t *d_pt_prefix, int 
 int64_t offsets_part_elems(int nfields, int64_t max_n);
 int launch_offsets(const FieldDev *d_fields, int nfields, int64_t max_n, const double *d_val, double *d_part,
                    double *d_offsets, hipStream_t stream);
-// one (base cell, point) pair per point; per-base-row point counts (row = x-line index) by
-// integer atomics unless d_row_count is NULL
+// one (base cell, point) pair per point and, unless d_row_count is NULL, per-base-row point
+// counts (row = x-line index) by integer atomics; with d_slot/d_rowg (row buckets) each point's
+// slot in its row and its row instead of the point indices
 int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
                       const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_row_count,
-                      hipStream_t stream);
+                      uint32_t *d_slot, uint32_t *d_rowg, hipStream_t stream);
+// fb_sort.hip -- row buckets for fields of more than kLocalSortMax points: scatter the in-grid
+// points into their rows (d_row_start from the scan of the counts), then sort each row by
+// (key, point index); d_svals/d_skeys receive the sorted list, d_tkeys/d_tvals are scratch for
+// rows of more than kLocalSortMax points; d_big_list holds 2*nrows entries and d_big_count 2
+// counters, which must be zero
+int row_bucket_sort(const uint32_t *d_keys, const uint32_t *d_slot, const uint32_t *d_rowg, int64_t npoints,
+                    const uint32_t *d_row_start, const uint32_t *d_row_count, int64_t nrows, uint32_t *d_skeys,
+                    uint32_t *d_svals, uint32_t *d_big_list, uint32_t *d_big_count, uint32_t *d_tkeys,
+                    uint32_t *d_tvals, hipStream_t stream);
 // per-base-row start and count from the sorted keys (both arrays zeroed by the caller)
 int launch_row_ranges(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfields, const uint32_t *d_keys,
                       int64_t n, uint32_t *d_row_start, uint32_t *d_row_count, int64_t nrows, hipStream_t stream);

@@ -136,6 +136,35 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const uint32_t *__re
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Bitonic network over sk[0, np2) (np2 a power of two) in LDS by the whole workgroup, ascending.
+// Pair t of a stage is (i, i + j) with i = t with a zero bit inserted at bit log2(j).  A thread
+// takes pairs t = tid + m*blockDim.x, so for j <= 64 wave w touches only the 128-element blocks
+// w + (waves per workgroup)*m: consecutive stages that both have j <= 64 depend on the wave's own
+// program order only; a barrier follows every stage where this stage or the next one has j > 64.
+// Ends with a workgroup barrier.
+__device__ __forceinline__ void block_bitonic(unsigned long long *sk, int np2)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int k = 2; k <= np2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = tid; t < (np2 >> 1); t += nt) {
+                const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+                const int l = i + j;
+                const unsigned long long a = sk[i], b = sk[l];
+                const bool up = (i & k) == 0;
+                if ((a > b) == up) {
+                    sk[i] = b;
+                    sk[l] = a;
+                }
+            }
+            const int jn = j > 1 ? (j >> 1) : k;   // the next stage's distance
+            if (j > 64 || jn > 64) __syncthreads();
+            else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+    }
+    __syncthreads();
+}
+
 // One workgroup per field: the field's (cell key, point index) pairs sorted in LDS by a
 // bitonic network on the 64-bit composite (equal keys stay in point order, as in the
 // stable radix sort), and the field's points inside the grid written in that order from
@@ -157,33 +186,183 @@ __global__ __launch_bounds__(kLocalSortThreads) void k_field_sort(const FieldDev
     for (int i = tid; i < np2; i += kLocalSortThreads)
         sk[i] = i < n ? ((unsigned long long)keys[p0 + i] << 32) | (unsigned long long)(uint32_t)(p0 + i) : ~0ull;
     __syncthreads();
-    // Pair t of a stage is (i, i + j) with i = t with a zero bit inserted at bit log2(j).  A
-    // thread takes pairs t = tid + m*kLocalSortThreads, so for j <= 64 wave w touches only the
-    // 128-element blocks w + 16m: consecutive stages that both have j <= 64 depend on the
-    // wave's own program order only; a barrier follows every stage where this stage or the
-    // next one has j > 64.
-    for (int k = 2; k <= np2; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = tid; t < (np2 >> 1); t += kLocalSortThreads) {
-                const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-                const int l = i + j;
-                const unsigned long long a = sk[i], b = sk[l];
-                const bool up = (i & k) == 0;
-                if ((a > b) == up) {
-                    sk[i] = b;
-                    sk[l] = a;
-                }
-            }
-            const int jn = j > 1 ? (j >> 1) : k;   // the next stage's distance
-            if (j > 64 || jn > 64) __syncthreads();
-            else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        }
-    }
-    __syncthreads();
+    block_bitonic(sk, np2);
     const uint32_t dst = row_start[fields[f].xline_off];
     for (int i = tid; i < n; i += kLocalSortThreads) {
         const unsigned long long e = sk[i];
         if ((uint32_t)(e >> 32) != kInvalidKey) out_vals[dst + i] = (uint32_t)e;
+    }
+}
+
+// ---- row buckets (fields of more than kLocalSortMax points) ------------------
+// The same order as the stable radix sort of (cell key, point index) pairs: every in-grid point
+// has a slot in its base row (k_point_keys' atomic count), the rows start at the exclusive scan
+// of the counts, and each row's entries are then sorted by the composite (key, point index).
+
+constexpr int kRowSmall = 16;           // rows of at most this many entries: one thread, in registers
+constexpr int kRowWave = 1024;          // rows of at most this many entries: one wave, in LDS
+constexpr int kRowSortThreads = 256;    // workgroup of the wave kernel (4 rows at a time) and of larger rows
+
+__device__ __forceinline__ unsigned long long row_entry(const uint32_t *keys, const uint32_t *vals, uint32_t i)
+{
+    return ((unsigned long long)keys[i] << 32) | (unsigned long long)vals[i];
+}
+
+__global__ __launch_bounds__(kThreads) void k_row_scatter(int64_t n, const uint32_t *__restrict__ keys,
+                                                          const uint32_t *__restrict__ slot,
+                                                          const uint32_t *__restrict__ rowg,
+                                                          const uint32_t *__restrict__ row_start,
+                                                          uint32_t *__restrict__ skeys, uint32_t *__restrict__ svals)
+{
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t r = rowg[i];
+    if (r == kInvalidKey) return;   // outside its grid
+    const uint32_t dst = row_start[r] + slot[i];
+    skeys[dst] = keys[i];
+    svals[dst] = (uint32_t)i;
+}
+
+// One thread per row: rows of 2..kRowSmall entries sorted by a bitonic network in registers
+// (padded with the largest composite); rows of up to kRowWave entries are appended to
+// lists[0] (count counts[0]), larger ones to lists[1] (counts[1]).
+__global__ __launch_bounds__(kThreads) void k_row_sort_small(int64_t nrows, const uint32_t *__restrict__ row_start,
+                                                             const uint32_t *__restrict__ row_count,
+                                                             uint32_t *__restrict__ skeys,
+                                                             uint32_t *__restrict__ svals,
+                                                             uint32_t *__restrict__ lists,
+                                                             uint32_t *__restrict__ counts)
+{
+    const int64_t r = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (r >= nrows) return;
+    const uint32_t n = row_count[r];
+    if (n <= 1) return;
+    if (n > (uint32_t)kRowSmall) {
+        const int c = n > (uint32_t)kRowWave;
+        lists[c * nrows + atomicAdd(&counts[c], 1u)] = (uint32_t)r;
+        return;
+    }
+    const uint32_t s0 = row_start[r];
+    unsigned long long e[kRowSmall];
+#pragma unroll
+    for (int q = 0; q < kRowSmall; ++q) e[q] = (uint32_t)q < n ? row_entry(skeys, svals, s0 + q) : ~0ull;
+#pragma unroll
+    for (int k = 2; k <= kRowSmall; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < kRowSmall; ++i) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const bool up = (i & k) == 0;
+                    const unsigned long long a = e[i], b = e[l];
+                    if ((a > b) == up) {
+                        e[i] = b;
+                        e[l] = a;
+                    }
+                }
+            }
+#pragma unroll
+    for (int q = 0; q < kRowSmall; ++q)
+        if ((uint32_t)q < n) {
+            skeys[s0 + q] = (uint32_t)(e[q] >> 32);
+            svals[s0 + q] = (uint32_t)e[q];
+        }
+}
+
+// Waves loop over the rows in list (at most kRowWave entries each): each wave sorts one row at a
+// time by the bitonic network in its own kRowWave-entry slice of LDS.  Within one wave every
+// stage only needs its LDS operations complete (a wavefront fence), never a workgroup barrier.
+__global__ __launch_bounds__(kRowSortThreads) void k_row_sort_wave(const uint32_t *__restrict__ row_start,
+                                                                   const uint32_t *__restrict__ row_count,
+                                                                   uint32_t *__restrict__ skeys,
+                                                                   uint32_t *__restrict__ svals,
+                                                                   const uint32_t *__restrict__ list,
+                                                                   const uint32_t *__restrict__ count)
+{
+    __shared__ unsigned long long lds[(kRowSortThreads / 64) * kRowWave];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    unsigned long long *sk = lds + wv * kRowWave;
+    const uint32_t nb = *count;
+    const uint32_t nwaves = gridDim.x * (kRowSortThreads / 64);
+    for (uint32_t b = blockIdx.x * (kRowSortThreads / 64) + wv; b < nb; b += nwaves) {
+        const uint32_t r = list[b];
+        const uint32_t n = row_count[r];
+        const uint32_t s0 = row_start[r];
+        int np2 = 1;
+        while ((uint32_t)np2 < n) np2 <<= 1;
+        for (int i = lane; i < np2; i += 64) sk[i] = (uint32_t)i < n ? row_entry(skeys, svals, s0 + i) : ~0ull;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int k = 2; k <= np2; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int t = lane; t < (np2 >> 1); t += 64) {
+                    const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+                    const int l = i + j;
+                    const unsigned long long a = sk[i], c = sk[l];
+                    const bool up = (i & k) == 0;
+                    if ((a > c) == up) {
+                        sk[i] = c;
+                        sk[l] = a;
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            }
+        for (int i = lane; (uint32_t)i < n; i += 64) {
+            const unsigned long long e = sk[i];
+            skeys[s0 + i] = (uint32_t)(e >> 32);
+            svals[s0 + i] = (uint32_t)e;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");   // before the next row reuses sk
+    }
+}
+
+// Workgroups loop over the rows in big_list (more than kRowWave entries): up to kLocalSortMax
+// entries by the bitonic network in LDS, more by rank (each entry counts the row's smaller composites; all composites differ)
+// through the scratch arrays tkeys/tvals.
+__global__ __launch_bounds__(kRowSortThreads) void k_row_sort_big(const uint32_t *__restrict__ row_start,
+                                                                  const uint32_t *__restrict__ row_count,
+                                                                  uint32_t *__restrict__ skeys,
+                                                                  uint32_t *__restrict__ svals,
+                                                                  const uint32_t *__restrict__ big_list,
+                                                                  const uint32_t *__restrict__ big_count,
+                                                                  uint32_t *__restrict__ tkeys,
+                                                                  uint32_t *__restrict__ tvals)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned long long *sk = reinterpret_cast<unsigned long long *>(smem);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const uint32_t nb = *big_count;
+    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const uint32_t r = big_list[b];
+        const uint32_t n = row_count[r];
+        const uint32_t s0 = row_start[r];
+        if (n <= (uint32_t)kLocalSortMax) {
+            int np2 = 1;
+            while ((uint32_t)np2 < n) np2 <<= 1;
+            for (int i = tid; i < np2; i += nt) sk[i] = (uint32_t)i < n ? row_entry(skeys, svals, s0 + i) : ~0ull;
+            __syncthreads();
+            block_bitonic(sk, np2);
+            for (int i = tid; (uint32_t)i < n; i += nt) {
+                const unsigned long long e = sk[i];
+                skeys[s0 + i] = (uint32_t)(e >> 32);
+                svals[s0 + i] = (uint32_t)e;
+            }
+        } else {
+            for (uint32_t i = tid; i < n; i += nt) {
+                const unsigned long long ei = row_entry(skeys, svals, s0 + i);
+                uint32_t rank = 0;
+                for (uint32_t j = 0; j < n; ++j) rank += row_entry(skeys, svals, s0 + j) < ei;
+                tkeys[s0 + rank] = (uint32_t)(ei >> 32);
+                tvals[s0 + rank] = (uint32_t)ei;
+            }
+            __syncthreads();
+            for (uint32_t i = tid; i < n; i += nt) {
+                skeys[s0 + i] = tkeys[s0 + i];
+                svals[s0 + i] = tvals[s0 + i];
+            }
+        }
+        __syncthreads();   // the next row reuses sk / reads what this one wrote
     }
 }
 
@@ -249,6 +428,31 @@ int radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_
 
 int local_sort_max() { return kLocalSortMax; }
 
+int row_bucket_sort(const uint32_t *d_keys, const uint32_t *d_slot, const uint32_t *d_rowg, int64_t npoints,
+                    const uint32_t *d_row_start, const uint32_t *d_row_count, int64_t nrows, uint32_t *d_skeys,
+                    uint32_t *d_svals, uint32_t *d_big_list, uint32_t *d_big_count, uint32_t *d_tkeys,
+                    uint32_t *d_tvals, hipStream_t stream)
+{
+    if (npoints <= 0 || nrows <= 0) return FB_OK;
+    hipLaunchKernelGGL(k_row_scatter, dim3((unsigned)cdiv(npoints, kThreads)), dim3(kThreads), 0, stream, npoints,
+                       d_keys, d_slot, d_rowg, d_row_start, d_skeys, d_svals);
+    FB_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_row_sort_small, dim3((unsigned)cdiv(nrows, kThreads)), dim3(kThreads), 0, stream, nrows,
+                       d_row_start, d_row_count, d_skeys, d_svals, d_big_list, d_big_count);
+    FB_HIP_CHECK(hipGetLastError());
+    const int64_t wgrid = std::min<int64_t>(cdiv(nrows, kRowSortThreads / 64), 2048);
+    hipLaunchKernelGGL(k_row_sort_wave, dim3((unsigned)wgrid), dim3(kRowSortThreads), 0, stream, d_row_start,
+                       d_row_count, d_skeys, d_svals, (const uint32_t *)d_big_list, (const uint32_t *)d_big_count);
+    FB_HIP_CHECK(hipGetLastError());
+    const int64_t grid = std::min<int64_t>(nrows, 1024);
+    hipLaunchKernelGGL(k_row_sort_big, dim3((unsigned)grid), dim3(kRowSortThreads),
+                       (size_t)kLocalSortMax * sizeof(unsigned long long), stream, d_row_start, d_row_count, d_skeys,
+                       d_svals, (const uint32_t *)(d_big_list + nrows), (const uint32_t *)(d_big_count + 1), d_tkeys,
+                       d_tvals);
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
 int field_sort_init()
 {
     static std::once_flag once;
@@ -256,6 +460,9 @@ int field_sort_init()
     std::call_once(once, [] {
         err = hipFuncSetAttribute((const void *)k_field_sort, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   kLocalSortMax * (int)sizeof(unsigned long long));
+        if (err == hipSuccess)
+            err = hipFuncSetAttribute((const void *)k_row_sort_big, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      kLocalSortMax * (int)sizeof(unsigned long long));
     });
     if (err != hipSuccess) {
         (void)hipGetLastError();

@@ -274,6 +274,55 @@ def test_blocked_intermediate_matches_row_major():
         _bitwise(a, b)
 
 
+def test_row_buckets_with_crowded_rows():
+    """Fields of more than 8192 points are sorted by row buckets: a slot per point in its base
+    row, then each row sorted by (cell, point index) -- by one thread (up to 16 entries), by a
+    workgroup in LDS (up to 8192) or by rank (more).  Crowded rows exercise the last two: a
+    field with 9000 points on one row and 1500 points on each of 3 rows, besides scattered
+    ones, one with every point on one row, and a tall sparse field (few points per row on
+    average: the plan sorts by row buckets).  The FP64 results must equal the oracle's bit for
+    bit (the reference's per-cell point order), and the FP32C path must agree with FP64."""
+    from fastbarnes.batched import BarnesBatch
+    from oracle import oracle as orc
+    rng = np.random.default_rng(77)
+    W, H, step = 300, 150, 0.25
+    x0 = np.array([-26.0, 34.0])
+
+    def on_rows(rows, n_per_row):
+        xs = rng.uniform(x0[0], x0[0] + (W - 1.2) * step, len(rows) * n_per_row)
+        ys = np.repeat([x0[1] + (r + 0.5) * step for r in rows], n_per_row)
+        return np.stack([xs, ys], axis=1)
+
+    p1 = np.concatenate([on_rows([70], 9000), on_rows([10, 11, 140], 1500),
+                         np.stack([rng.uniform(-26, 48, 2000), rng.uniform(34, 71, 2000)], axis=1)])
+    p1 = p1[rng.permutation(len(p1))]
+    p2 = on_rows([33], 12000)
+    cases = []
+    for p in (p1, p2):
+        cases.append(dict(pts=p, val=np.sin(p[:, 0] / 3.0) * 10 + p[:, 1], sigma=1.2, x0=x0, step=step,
+                          size=(W, H), num_iter=4, max_dist=3.5))
+    # a tall sparse field keeps the batch at <= 8 points per row on average, so that the plan
+    # sorts by row buckets rather than by the radix sort
+    p3 = np.stack([rng.uniform(x0[0], x0[0] + 70, 300), rng.uniform(x0[1], x0[1] + 990, 300)], axis=1)
+    cases.append(dict(pts=p3, val=np.cos(p3[:, 1] / 7.0), sigma=1.2, x0=x0, step=step, size=(W, 4000), num_iter=4,
+                      max_dist=3.5))
+    rows = sum(c['size'][1] for c in cases)
+    assert sum(len(c['val']) for c in cases) <= 8 * rows
+    for prec in ('fp64', 'fp32c'):
+        batch = BarnesBatch([_desc(c) for c in cases], precision=prec)
+        outs = batch.interpolate([c['pts'] for c in cases], [c['val'] for c in cases])
+        for c, o in zip(cases, outs):
+            ref = orc.interpolate_convol(c['pts'], c['val'], c['sigma'], c['x0'], c['step'], c['size'],
+                                         num_iter=c['num_iter'], max_dist=c['max_dist'])
+            if prec == 'fp64':
+                _bitwise(o, ref)
+            else:
+                np.testing.assert_array_equal(np.isnan(o), np.isnan(ref))
+                ok = ~np.isnan(ref)
+                d = o[ok].astype(np.float64) - ref[ok]
+                assert float(np.sqrt(np.mean(d * d))) <= 1e-5 * float(np.ptp(c['val']))
+
+
 def test_choose_pass_rejects_bad_choices():
     from fastbarnes.batched import BarnesBatch
     c = _fields(3)[0]
