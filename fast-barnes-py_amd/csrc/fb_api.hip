@@ -410,7 +410,6 @@ struct fb_plan {
     int precision = FB_PREC_FP64;
     int nfields = 0, dim = 0, corners = 0;
     int64_t npoints = 0, ncells = 0, nxlines = 0, nentries = 0, total_out = 0, total_cells = 0;
-    int key_bits = 8;
     std::vector<FieldDev> fh;
     std::vector<DevBuf> bufs;
     size_t ws_bytes = 0;
@@ -422,9 +421,8 @@ struct fb_plan {
     double *d_offsets = nullptr, *d_off_part = nullptr;
     int64_t max_n = 0;
     uint32_t *d_keys = nullptr, *d_vals = nullptr, *d_keys2 = nullptr, *d_vals2 = nullptr;
-    uint32_t *d_line_count = nullptr, *d_line_start = nullptr, *d_hist = nullptr, *d_scan_tmp = nullptr;
+    uint32_t *d_line_count = nullptr, *d_line_start = nullptr, *d_scan_tmp = nullptr;
     uint32_t *d_skeys = nullptr, *d_big_list = nullptr, *d_big_count = nullptr;   // row buckets
-    bool row_buckets = false;
     double *d_vg = nullptr, *d_wg = nullptr, *d_vs = nullptr, *d_ws = nullptr;
     float *d_v = nullptr, *d_w = nullptr;
     PassWork *d_work[3] = {nullptr, nullptr, nullptr};
@@ -727,9 +725,6 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
         delete P;
         return fail(FB_ERR_UNSUPPORTED, "a plan holds at most 2^32-2 grid cells and 2^32-1 contributions");
     }
-    int bits = 1;
-    while (bits < 32 && ((int64_t)1 << bits) <= P->ncells) ++bits;
-    P->key_bits = (bits + 7) / 8 * 8;
 
     int rc = FB_OK;
 #define TRY(x) do { rc = (x); if (rc) { delete P; return rc; } } while (0)
@@ -745,20 +740,14 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
     TRY(P->alloc(&P->d_vals, P->nentries));
     TRY(P->alloc(&P->d_keys2, P->nentries));
     TRY(P->alloc(&P->d_vals2, P->nentries));
-    // row buckets for plans of large fields with at most 8 points per row on average (cfg4-like
-    // 3D batches: 3-4); crowded rows (13 and 83 per row in the measured 2D batches) make the
-    // per-row atomic counters contend, where the radix sort needs none
-    P->row_buckets = P->max_n > kLocalSortMax && P->npoints <= 8 * P->nxlines;
-    if (P->row_buckets) {
+    if (P->max_n > kLocalSortMax) {
         TRY(P->alloc(&P->d_skeys, P->nentries));
         TRY(P->alloc(&P->d_big_list, 2 * P->nxlines));   // rows for the wave kernel, then larger rows
         TRY(P->alloc(&P->d_big_count, 2));
     }
     TRY(P->alloc(&P->d_line_count, P->nxlines));
     TRY(P->alloc(&P->d_line_start, P->nxlines));
-    const int64_t nh = radix_hist_elems(P->nentries);
-    TRY(P->alloc(&P->d_hist, nh));
-    TRY(P->alloc(&P->d_scan_tmp, scan_temp_elems(std::max(nh, P->nxlines))));
+    TRY(P->alloc(&P->d_scan_tmp, scan_temp_elems(P->nxlines)));
     if (!fast) {
         TRY(P->alloc(&P->d_vg, P->ncells));
         TRY(P->alloc(&P->d_wg, P->ncells));
@@ -791,44 +780,30 @@ int plan_run(fb_plan *P, const double *pts, const double *val, float *out, float
     P->events.clear();
     P->stage_names.clear();
     stage_mark(P, s, "begin");
-    // Three ways to the sorted list, all in the order of a stable sort of the (key, point index)
-    // pairs:
-    // * fields of at most kLocalSortMax points: per-row counts by integer atomics, their scan,
-    //   then one workgroup sorts each field in LDS;
-    // * larger fields with few points per row: the same counts and scan, every point into its
-    //   slot of its row's range, then each row sorted by (key, point index);
-    // * larger fields with crowded rows (where the counters' atomics contend): the LSD radix sort
-    //   over the batch, then the row ranges from the sorted keys (no atomics, no scan).
+    // Per-row point counts by integer atomics and their exclusive scan give the rows' ranges of
+    // the sorted list.  Fields of at most kLocalSortMax points: one workgroup sorts each field
+    // in LDS.  Larger fields: every point goes to its atomic slot of its row's range, then each
+    // row is sorted by (key, point index).  Both give the order of a stable sort of the (key,
+    // point index) pairs.
     const bool local = P->max_n <= kLocalSortMax;
-    const bool buckets = !local && P->row_buckets;
-    const bool counts = local || buckets;
     if ((rc = launch_zero(P->d_line_count, P->nxlines * 4, s))) return rc;
-    if (!counts && (rc = launch_zero(P->d_line_start, P->nxlines * 4, s))) return rc;
-    if (buckets && (rc = launch_zero(P->d_big_count, 8, s))) return rc;
+    if (!local && (rc = launch_zero(P->d_big_count, 8, s))) return rc;
     if ((rc = launch_offsets(P->d_fields, P->nfields, P->max_n, val, P->d_off_part, P->d_offsets, s))) return rc;
     if ((rc = launch_point_keys(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints, pts, P->d_keys, P->d_vals,
-                                counts ? P->d_line_count : nullptr, buckets ? P->d_vals : nullptr,
-                                buckets ? P->d_keys2 : nullptr, s)))
+                                P->d_line_count, local ? nullptr : P->d_vals, local ? nullptr : P->d_keys2, s)))
         return rc;
-    uint32_t *ks = nullptr, *vs = P->d_vals2;
-    if (counts && (rc = scan_exclusive(P->d_line_count, P->d_line_start, P->nxlines, P->d_scan_tmp, s))) return rc;
+    if ((rc = scan_exclusive(P->d_line_count, P->d_line_start, P->nxlines, P->d_scan_tmp, s))) return rc;
+    uint32_t *vs = P->d_vals2;
     if (local) {
         if ((rc = launch_field_sort(P->d_fields, P->d_pt_prefix, P->nfields, (int)P->max_n, P->d_keys,
                                     P->d_line_start, P->d_vals2, s)))
             return rc;
-    } else if (buckets) {
+    } else {
         // d_vals holds the slots and d_keys2 the rows; both, with d_keys, are free after the
         // scatter and serve as scratch for rows of more than kLocalSortMax points
         if ((rc = row_bucket_sort(P->d_keys, P->d_vals, P->d_keys2, P->npoints, P->d_line_start, P->d_line_count,
                                   P->nxlines, P->d_skeys, P->d_vals2, P->d_big_list, P->d_big_count, P->d_keys,
                                   P->d_vals, s)))
-            return rc;
-    } else {
-        if ((rc = radix_sort_pairs(P->d_keys, P->d_vals, P->d_keys2, P->d_vals2, P->nentries, P->key_bits,
-                                   P->d_hist, P->d_scan_tmp, s, &ks, &vs)))
-            return rc;
-        if ((rc = launch_row_ranges(P->d_fields, P->d_cell_prefix, P->nfields, ks, P->nentries, P->d_line_start,
-                                    P->d_line_count, P->nxlines, s)))
             return rc;
     }
     P->sorted_vals = vs;

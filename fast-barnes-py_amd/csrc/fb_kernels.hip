@@ -4,7 +4,8 @@
 //   k_offsets       value offset (min+max)/2 per field           interpolation.py:204-212
 //   k_point_keys    per point: base cell (lower corner) -> (cell key, point) pair,
 //                   per-row point counts                          interpolation.py:218-322
-//   (fb_sort.hip)   stable radix sort of the pairs by base cell, scan of the row counts
+//   (fb_sort.hip)   scan of the row counts; per-field LDS sort or row buckets ordering the
+//                   pairs by (base cell, point index)
 //   FP64 path:      k_exact_inject, k_exact_lines (per axis), k_exact_finalize
 //                   -- the reference's operation order, one thread per line
 //   FP32C path:     k_fast_pass<inject> along x, k_fast_pass along y (and z);
@@ -200,40 +201,6 @@ __global__ __launch_bounds__(kBlock) void k_point_keys(const FieldDev *__restric
             rowg[i] = (uint32_t)(F.xline_off + row);
         }
     }
-}
-
-// Global x-line (base row) of a sorted key inside the grid.
-__device__ __forceinline__ int64_t key_row(const FieldDev *__restrict__ fields, const int64_t *__restrict__ cell_prefix,
-                                           int nfields, uint32_t k)
-{
-    const FieldDev &F = fields[find_seg(cell_prefix, nfields, (int64_t)k)];
-    return F.xline_off + ((int64_t)k - F.cell_off) / F.size[0];
-}
-
-// Row ranges from the sorted keys: the first entry of a row's run stores its position in
-// row_start, the last one its end in row_end (rows without points keep the zeros they were
-// given).  No atomics, no scan.
-__global__ __launch_bounds__(kBlock) void k_row_bounds(const FieldDev *__restrict__ fields,
-                                                       const int64_t *__restrict__ cell_prefix, int nfields,
-                                                       const uint32_t *__restrict__ keys, int64_t n,
-                                                       uint32_t *__restrict__ row_start, uint32_t *__restrict__ row_end)
-{
-    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (p >= n) return;
-    const uint32_t k = keys[p];
-    if (k == kInvalidKey) return;
-    const int64_t r = key_row(fields, cell_prefix, nfields, k);
-    if (p == 0 || key_row(fields, cell_prefix, nfields, keys[p - 1]) != r) row_start[r] = (uint32_t)p;
-    const uint32_t kn = p + 1 < n ? keys[p + 1] : kInvalidKey;
-    if (kn == kInvalidKey || key_row(fields, cell_prefix, nfields, kn) != r) row_end[r] = (uint32_t)(p + 1);
-}
-
-// row_count[r] = row_end[r] - row_start[r], in place (row_count holds the ends on entry)
-__global__ __launch_bounds__(kBlock) void k_row_counts(const uint32_t *__restrict__ row_start,
-                                                       uint32_t *__restrict__ row_count, int64_t nrows)
-{
-    const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (r < nrows) row_count[r] -= row_start[r];
 }
 
 // The sub-streams of x-line (y, z): for q < 2*nrows, the points of base row (y - dy, z - dz)
@@ -1538,21 +1505,6 @@ int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int 
     if (npoints <= 0) return FB_OK;
     hipLaunchKernelGGL(k_point_keys, dim3(nblocks(npoints)), dim3(kBlock), 0, s, d_fields, d_pt_prefix, nfields,
                        npoints, d_pts, d_keys, d_vals, d_row_count, d_slot, d_rowg);
-    FB_HIP_CHECK(hipGetLastError());
-    return FB_OK;
-}
-
-int launch_row_ranges(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfields, const uint32_t *d_keys,
-                      int64_t n, uint32_t *d_row_start, uint32_t *d_row_count, int64_t nrows, hipStream_t s)
-{
-    if (nrows <= 0) return FB_OK;
-    if (n > 0) {
-        hipLaunchKernelGGL(k_row_bounds, dim3(nblocks(n)), dim3(kBlock), 0, s, d_fields, d_cell_prefix, nfields, d_keys,
-                           n, d_row_start, d_row_count);
-        FB_HIP_CHECK(hipGetLastError());
-    }
-    hipLaunchKernelGGL(k_row_counts, dim3(nblocks(nrows)), dim3(kBlock), 0, s, (const uint32_t *)d_row_start,
-                       d_row_count, nrows);
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }

@@ -9,12 +9,8 @@ namespace fb {
 // fb_sort.hip
 int64_t scan_temp_elems(int64_t n);
 int scan_exclusive(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tmp, hipStream_t stream);
-int64_t radix_hist_elems(int64_t n);
-int radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, int64_t n, int bits,
-                     uint32_t *hist, uint32_t *scan_tmp, hipStream_t stream, uint32_t **keys_out,
-                     uint32_t **vals_out);
 // per-field sort in LDS (fields of at most kLocalSortMax points): the sorted in-grid points
-// of field f from out_vals[row_start[first x-line of f]] on; the same order as the radix sort
+// of field f from out_vals[row_start[first x-line of f]] on; the same order as the row buckets
 int field_sort_init();
 int launch_field_sort(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int max_n,
                       const uint32_t *d_keys, const uint32_t *d_row_start, uint32_t *d_out_vals, hipStream_t stream);
@@ -40,9 +36,6 @@ int row_bucket_sort(const uint32_t *d_keys, const uint32_t *d_slot, const uint32
                     const uint32_t *d_row_start, const uint32_t *d_row_count, int64_t nrows, uint32_t *d_skeys,
                     uint32_t *d_svals, uint32_t *d_big_list, uint32_t *d_big_count, uint32_t *d_tkeys,
                     uint32_t *d_tvals, hipStream_t stream);
-// per-base-row start and count from the sorted keys (both arrays zeroed by the caller)
-int launch_row_ranges(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfields, const uint32_t *d_keys,
-                      int64_t n, uint32_t *d_row_start, uint32_t *d_row_count, int64_t nrows, hipStream_t stream);
 
 // fb_kernels.hip -- FP64 path with the reference's operation order
 int launch_exact_inject(const FieldDev *d_fields, const int64_t *d_xline_prefix, int nfields, int64_t nxlines,

@@ -1,10 +1,10 @@
-// fb_sort.hip -- exclusive scan, stable LSD radix sort of (key, value) uint32 pairs, and a
-// per-field sort in LDS for batches of small fields.
+// fb_sort.hip -- exclusive scan, the per-field sort in LDS (fields of at most kLocalSortMax
+// points) and the row-bucket sort (larger fields).
 //
-// Used to order the injection contributions by grid cell while keeping, inside a
-// cell, the point order of the input (the order in which the reference's
-// _inject_data_*d loop adds them, interpolation.py:246-270).  Histogram counts use
-// integer LDS atomics only, so the result is deterministic.
+// Used to order the injection's points by base cell while keeping, inside a cell, the point
+// order of the input (the order in which the reference's _inject_data_*d loop adds them,
+// interpolation.py:246-270): both sorts order the composite (cell key, point index), so the
+// result does not depend on timing.
 #include <mutex>
 
 #include "fb_internal.h"
@@ -17,12 +17,9 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kItems = 16;
 constexpr int kTile = kThreads * kItems;   // 4096 elements per workgroup of the multi-level scan
-constexpr int kRItems = 4;
-constexpr int kRTile = kThreads * kRItems; // 1024 elements per workgroup of the radix passes
-constexpr int kRadixBits = 8;
-constexpr int kRadix = 1 << kRadixBits;
-constexpr int kWaves = kThreads / 64;
 constexpr int kLocalSortThreads = 1024;
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // ---- scan ----------------------------------------------------------------
 
@@ -66,76 +63,6 @@ __global__ __launch_bounds__(kThreads) void k_scan_add(uint32_t *__restrict__ ou
     if (idx < n) out[idx] += tile_offs[idx / kTile];
 }
 
-// ---- radix sort ----------------------------------------------------------
-
-__global__ __launch_bounds__(kThreads) void k_radix_hist(const uint32_t *__restrict__ keys, int64_t n, int shift,
-                                                         uint32_t *__restrict__ hist, int ntiles)
-{
-    __shared__ uint32_t h[kRadix];
-    const int tid = threadIdx.x;
-    h[tid] = 0u;
-    __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kRTile;
-    for (int r = 0; r < kRItems; ++r) {
-        const int64_t idx = base + (int64_t)r * kThreads + tid;
-        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (kRadix - 1)], 1u);
-    }
-    __syncthreads();
-    hist[(int64_t)tid * ntiles + blockIdx.x] = h[tid];
-}
-
-__global__ __launch_bounds__(kThreads) void k_radix_scatter(const uint32_t *__restrict__ kin,
-                                                            const uint32_t *__restrict__ vin,
-                                                            uint32_t *__restrict__ kout,
-                                                            uint32_t *__restrict__ vout, int64_t n, int shift,
-                                                            const uint32_t *__restrict__ offs, int ntiles)
-{
-    __shared__ uint32_t running[kRadix];
-    __shared__ uint32_t wcnt[kWaves][kRadix];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const uint64_t lanes_below = lane ? (~0ull >> (64 - lane)) : 0ull;
-    running[tid] = offs[(int64_t)tid * ntiles + blockIdx.x];
-    const int64_t base = (int64_t)blockIdx.x * kRTile;
-    for (int r = 0; r < kRItems; ++r) {
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0u;
-        __syncthreads();
-        const int64_t idx = base + (int64_t)r * kThreads + tid;
-        const bool valid = idx < n;
-        const uint32_t key = valid ? kin[idx] : 0u;
-        const uint32_t val = valid ? vin[idx] : 0u;
-        const uint32_t d = (key >> shift) & (kRadix - 1);
-        // lanes of this wave holding the same digit (stable rank = lanes below with the same digit)
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < kRadixBits; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bal = __ballot(bit);
-            peers &= bit ? bal : ~bal;
-        }
-        const uint32_t rank = (uint32_t)__popcll(peers & lanes_below);
-        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t pre = 0;
-            for (int w = 0; w < wave; ++w) pre += wcnt[w][d];
-            const uint32_t pos = running[d] + pre + rank;
-            kout[pos] = key;
-            vout[pos] = val;
-        }
-        __syncthreads();
-        uint32_t add = 0;
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) add += wcnt[w][tid];
-        running[tid] += add;
-        __syncthreads();
-    }
-}
-
-inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
-
 // Bitonic network over sk[0, np2) (np2 a power of two) in LDS by the whole workgroup, ascending.
 // Pair t of a stage is (i, i + j) with i = t with a zero bit inserted at bit log2(j).  A thread
 // takes pairs t = tid + m*blockDim.x, so for j <= 64 wave w touches only the 128-element blocks
@@ -167,7 +94,7 @@ __device__ __forceinline__ void block_bitonic(unsigned long long *sk, int np2)
 
 // One workgroup per field: the field's (cell key, point index) pairs sorted in LDS by a
 // bitonic network on the 64-bit composite (equal keys stay in point order, as in the
-// stable radix sort), and the field's points inside the grid written in that order from
+// row-bucket sort), and the field's points inside the grid written in that order from
 // its first row start on.  Fields of at most kLocalSortMax points.
 __global__ __launch_bounds__(kLocalSortThreads) void k_field_sort(const FieldDev *__restrict__ fields,
                                                                   const int64_t *__restrict__ pt_prefix,
@@ -195,7 +122,7 @@ __global__ __launch_bounds__(kLocalSortThreads) void k_field_sort(const FieldDev
 }
 
 // ---- row buckets (fields of more than kLocalSortMax points) ------------------
-// The same order as the stable radix sort of (cell key, point index) pairs: every in-grid point
+// The order of a stable sort of the (cell key, point index) pairs: every in-grid point
 // has a slot in its base row (k_point_keys' atomic count), the rows start at the exclusive scan
 // of the counts, and each row's entries are then sorted by the composite (key, point index).
 
@@ -234,14 +161,24 @@ __global__ __launch_bounds__(kThreads) void k_row_sort_small(int64_t nrows, cons
                                                              uint32_t *__restrict__ counts)
 {
     const int64_t r = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (r >= nrows) return;
-    const uint32_t n = row_count[r];
-    if (n <= 1) return;
-    if (n > (uint32_t)kRowSmall) {
-        const int c = n > (uint32_t)kRowWave;
-        lists[c * nrows + atomicAdd(&counts[c], 1u)] = (uint32_t)r;
-        return;
+    const uint32_t n = r < nrows ? row_count[r] : 0;
+    // list appends aggregated per wave: one atomic per wave and list (a single counter taking
+    // one atomic per row serialises when most rows are large)
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const bool mine = c == 0 ? (n > (uint32_t)kRowSmall && n <= (uint32_t)kRowWave) : n > (uint32_t)kRowWave;
+        const unsigned long long m = __ballot(mine);
+        if (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&counts[c], (uint32_t)__popcll(m));
+            base = __shfl(base, leader);
+            if (mine) lists[c * nrows + base + (uint32_t)__popcll(m & below)] = (uint32_t)r;
+        }
     }
+    if (n <= 1 || n > (uint32_t)kRowSmall) return;
     const uint32_t s0 = row_start[r];
     unsigned long long e[kRowSmall];
 #pragma unroll
@@ -395,34 +332,6 @@ int scan_exclusive(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tmp, 
     hipLaunchKernelGGL(k_scan_add, dim3((unsigned)cdiv(n, kThreads)), dim3(kThreads), 0, stream, out,
                        (const uint32_t *)sums, n);
     FB_HIP_CHECK(hipGetLastError());
-    return FB_OK;
-}
-
-int64_t radix_hist_elems(int64_t n) { return (int64_t)kRadix * cdiv(n > 0 ? n : 1, kRTile); }
-
-int radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, int64_t n, int bits,
-                     uint32_t *hist, uint32_t *scan_tmp, hipStream_t stream, uint32_t **keys_out,
-                     uint32_t **vals_out)
-{
-    uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
-    if (n > 0) {
-        const int ntiles = (int)cdiv(n, kRTile);
-        const int64_t nh = (int64_t)kRadix * ntiles;
-        for (int shift = 0; shift < bits; shift += kRadixBits) {
-            hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kThreads), 0, stream, (const uint32_t *)ki, n,
-                               shift, hist, ntiles);
-            FB_HIP_CHECK(hipGetLastError());
-            int rc = scan_exclusive(hist, hist, nh, scan_tmp, stream);
-            if (rc) return rc;
-            hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kThreads), 0, stream, (const uint32_t *)ki,
-                               (const uint32_t *)vi, ko, vo, n, shift, (const uint32_t *)hist, ntiles);
-            FB_HIP_CHECK(hipGetLastError());
-            uint32_t *t = ki; ki = ko; ko = t;
-            t = vi; vi = vo; vo = t;
-        }
-    }
-    *keys_out = ki;
-    *vals_out = vi;
     return FB_OK;
 }
 
