@@ -877,11 +877,14 @@ __device__ __forceinline__ void inject_line(const FieldDev &F, int64_t line_in_f
         const bool start = sid == 0 || px != x;
         const int lane = threadIdx.x & 63;
         const int rid = __popcll(__ballot(start) & ((2ull << lane) - 1ull));
-        for (int d = 1; d < iw; d <<= 1) {
-            const double oc = __shfl_down(c, d, iw);
-            const int orid = __shfl_down(rid, d, iw);
-            if (sid + d < iw && orid == rid) c += oc;
-        }
+        // the segmented sum only where some run of this wave's chunk has more than one entry
+        // (with sparse stations, rarely): otherwise every lane is its own run and keeps c
+        if (__ballot(valid && !start))
+            for (int d = 1; d < iw; d <<= 1) {
+                const double oc = __shfl_down(c, d, iw);
+                const int orid = __shfl_down(rid, d, iw);
+                if (sid + d < iw && orid == rid) c += oc;
+            }
         const bool head = valid && start;
         float *cell = row + (x < 0 ? 0 : x);
         for (int q = 0; q < nsub; ++q)
