@@ -209,19 +209,30 @@ __global__ __launch_bounds__(kBlock) void k_point_keys(const FieldDev *__restric
 // x), so every sub-stream ascends in target cell.
 __device__ __forceinline__ int source_rows(int dim) { return dim >= 3 ? 4 : (dim == 2 ? 2 : 1); }
 
-__device__ __forceinline__ bool sub_stream(const FieldDev &F, int64_t line, int q,
-                                           const uint32_t *__restrict__ row_start,
-                                           const uint32_t *__restrict__ row_count, uint32_t &rs, uint32_t &rc)
+__device__ __forceinline__ void line_streams(const FieldDev &F, int64_t line, bool enable,
+                                             const uint32_t *__restrict__ row_start,
+                                             const uint32_t *__restrict__ row_count, uint32_t (&rs)[8],
+                                             uint32_t (&rc)[8])
 {
-    const int dy = (q >> 1) & 1, dz = q >> 2;
+    // sub-stream q reads base row (y - dy, z - dz) with dy = (q >> 1) & 1, dz = q >> 2 for its
+    // corner column bx = q & 1: both corner columns of a row share the row's range, loaded
+    // once; (y, z) are formed once per line (a 2D line index is y itself)
+    const int nsub = 2 * source_rows(F.dim);
     const int64_t H = F.size[1];
-    const int64_t y = line % H, z = line / H;
-    rs = rc = 0;
-    if (y - dy < 0 || z - dz < 0) return false;
-    const int64_t row = line - dy - dz * H;
-    rs = row_start[F.xline_off + row];
-    rc = row_count[F.xline_off + row];
-    return true;
+    const int64_t y = F.dim >= 3 ? line % H : line;
+    const int64_t z = F.dim >= 3 ? line / H : 0;
+#pragma unroll
+    for (int q = 0; q < 8; q += 2) {
+        const int dy = (q >> 1) & 1, dz = q >> 2;
+        uint32_t st = 0, ct = 0;
+        if (enable && q < nsub && y - dy >= 0 && z - dz >= 0) {
+            const int64_t row = line - dy - dz * H;
+            st = row_start[F.xline_off + row];
+            ct = row_count[F.xline_off + row];
+        }
+        rs[q] = rs[q + 1] = st;
+        rc[q] = rc[q + 1] = ct;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -243,18 +254,16 @@ __global__ __launch_bounds__(kBlock) void k_exact_inject(
     const int64_t line = gl - F.xline_off;
     const int64_t base = F.cell_off + line * F.size[0];
     const double off = offsets[f];
-    const int nsub = 2 * source_rows(F.dim);
     uint32_t pos[8], end[8], hp[8];
     int hx[8];
     double hw[8];
+    {
+        uint32_t rs[8], rc[8];
+        line_streams(F, line, true, row_start, row_count, rs, rc);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        pos[q] = end[q] = 0;
-        if (q < nsub) {
-            uint32_t rs, rc;
-            sub_stream(F, line, q, row_start, row_count, rs, rc);
-            pos[q] = rs;
-            end[q] = rs + rc;
+        for (int q = 0; q < 8; ++q) {
+            pos[q] = rs[q];
+            end[q] = rs[q] + rc[q];
         }
     }
     // head of sub-stream q: target x, point, corner weight (INT_MAX when exhausted)
@@ -824,7 +833,7 @@ __device__ __forceinline__ void adjoint_load(const float *__restrict__ f, const 
 }
 
 // Deterministic fused injection of one line's points into its float32 row (element x at
-// row[x]).  The line's sub-streams (sub_stream: one base row, one corner column),
+// row[x]).  The line's sub-streams (line_streams: one base row, one corner column),
 // concatenated, are taken in chunks of up to 64 entries by the line's first `iw` lanes
 // (`injector`).  A segmented suffix sum over adjacent lanes of equal target cell (fixed
 // tree order) gives each run's total to its first lane.  Equal cells adjacent in one
@@ -839,15 +848,11 @@ __device__ __forceinline__ void inject_line(const FieldDev &F, int64_t line_in_f
                                             const double *__restrict__ val, float *row)
 {
     const int nsub = 2 * source_rows(F.dim);
-    uint32_t rsq[8], pre[9];
+    uint32_t rsq[8], rcq[8], pre[9];
+    line_streams(F, line_in_field, injector, line_start, line_count, rsq, rcq);
     pre[0] = 0;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        uint32_t rs = 0, rc = 0;
-        if (injector && q < nsub) sub_stream(F, line_in_field, q, line_start, line_count, rs, rc);
-        rsq[q] = rs;
-        pre[q + 1] = pre[q] + rc;
-    }
+    for (int q = 0; q < 8; ++q) pre[q + 1] = pre[q] + rcq[q];
     int nch = (int)((pre[8] + iw - 1) / iw);
     for (int d = 32; d >= 1; d >>= 1) nch = max(nch, __shfl_xor(nch, d));
     for (int ch = 0; ch < nch; ++ch) {
