@@ -601,10 +601,22 @@ __device__ __forceinline__ bool aligned16(const void *p) { return ((uintptr_t)p 
 // to lds[l*lstride + j].  Either ls == 1 (columns: rows of nl contiguous floats) or es == 1
 // (rows of L contiguous floats); float4 global accesses when the layout allows.  All
 // threads of the workgroup take part.  Pointers advance by increments (no 64-bit index
-// products per element).
-template <bool LOAD>
+// products per element).  A store takes the value of each LDS element through `src`: the
+// element itself, or the masked quotient of the v element and the w element `wdiff` floats
+// after it (the last pass's result, formed on the way out).
+struct LdsValue {
+    __device__ __forceinline__ float operator()(const float *d) const { return *d; }
+};
+struct LdsQuotient {
+    int wdiff;
+    double off;
+    float thr_f;
+    __device__ __forceinline__ float operator()(const float *d) const { return fast_quotient(d[0], d[wdiff], off, thr_f); }
+};
+
+template <bool LOAD, class SRC = LdsValue>
 __device__ __forceinline__ void tile_copy(float *__restrict__ g, int64_t ls, int64_t es, int nl, int L,
-                                          float *__restrict__ lds, int lstride)
+                                          float *__restrict__ lds, int lstride, SRC src = SRC())
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
     if (ls == 1) {
@@ -626,7 +638,8 @@ __device__ __forceinline__ void tile_copy(float *__restrict__ g, int64_t ls, int
                         d[2 * lstride] = x.z;
                         d[3 * lstride] = x.w;
                     } else {
-                        *reinterpret_cast<float4 *>(gp) = make_float4(d[0], d[lstride], d[2 * lstride], d[3 * lstride]);
+                        *reinterpret_cast<float4 *>(gp) =
+                            make_float4(src(d), src(d + lstride), src(d + 2 * lstride), src(d + 3 * lstride));
                     }
                 }
             }
@@ -641,7 +654,7 @@ __device__ __forceinline__ void tile_copy(float *__restrict__ g, int64_t ls, int
 #pragma unroll 4
                 for (int r = r0; r < L; r += rstep, gp += gstep, d += rstep) {
                     if (LOAD) *d = *gp;
-                    else *gp = *d;
+                    else *gp = src(d);
                 }
             }
         }
@@ -662,7 +675,7 @@ __device__ __forceinline__ void tile_copy(float *__restrict__ g, int64_t ls, int
                         dd[2] = x.z;
                         dd[3] = x.w;
                     } else {
-                        *gp = make_float4(dd[0], dd[1], dd[2], dd[3]);
+                        *gp = make_float4(src(dd), src(dd + 1), src(dd + 2), src(dd + 3));
                     }
                 }
             }
@@ -674,7 +687,7 @@ __device__ __forceinline__ void tile_copy(float *__restrict__ g, int64_t ls, int
 #pragma unroll 4
                 for (int j = tid; j < L; j += nthr, gp += gstep, d += nthr) {
                     if (LOAD) *d = *gp;
-                    else *gp = *d;
+                    else *gp = src(d);
                 }
             }
         }
@@ -1009,15 +1022,9 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     }
 
     if (final_pass) {
-        // masked quotient + offset (interpolation.py:367, :428-430), in place in the v lines
-        const float thr_f = F.thr_fast_f;
-        for (int l = 0; l < nl; ++l)
-            for (int j = tid; j < L; j += nthr) {
-                const int o = l * lstride + pad + j;
-                sv[o] = fast_quotient(sv[o], sw[o], off, thr_f);
-            }
-        __syncthreads();
-        tile_copy<false>(out + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
+        // masked quotient + offset (interpolation.py:367, :428-430), formed as the v lines are stored
+        tile_copy<false>(out + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride,
+                         LdsQuotient{(int)(sw - sv), off, F.thr_fast_f});
         // training forward: keep the normalised weight field for the adjoint
         if (wsave) tile_copy<false>(wsave + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
     } else if (P.blocked == kBlockedOut) {
@@ -1302,15 +1309,9 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_prefix_pass(
     }
 
     if (final_pass) {
-        // masked quotient + offset (interpolation.py:367, :428-430), in place in the v tiles
-        const float thr_f = F.thr_fast_f;
-        for (int l = 0; l < nl; ++l)
-            for (int j = tid; j < L; j += nthr) {
-                const int o = l * fstride + j;
-                tv[o] = fast_quotient(tv[o], tw[o], off, thr_f);
-            }
-        __syncthreads();
-        tile_copy<false>(out + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, tv, fstride);
+        // masked quotient + offset (interpolation.py:367, :428-430), formed as the v tiles are stored
+        tile_copy<false>(out + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, tv, fstride,
+                         LdsQuotient{(int)(tw - tv), off, F.thr_fast_f});
         if (wsave) tile_copy<false>(wsave + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, tw, fstride);
     } else {
         tile_copy<false>(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, tv, fstride);
