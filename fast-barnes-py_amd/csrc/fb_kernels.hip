@@ -910,6 +910,15 @@ __device__ __forceinline__ void inject_line(const FieldDev &F, int64_t line_in_f
     }
 }
 
+// Orders the LDS accesses of one phase of a line pass before the next: a workgroup
+// barrier when a line spans several waves; a line within one wave needs only program order
+// (a wave's LDS instructions execute in order), kept by a compiler fence.
+__device__ __forceinline__ void line_sync(int lpl)
+{
+    if (lpl > 64) __syncthreads();
+    else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
 // One workgroup = 128, 256 or 512 threads: the first half works on the v field, the second
 // half on the w field of the same nlines lines (each half: up to lines_cap lines of lpl
 // lanes).  LDS holds the v lines, then the w lines, each padded by zeros on both sides.
@@ -1002,6 +1011,10 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     const int over = (kSegArr - L % kSegArr) % kSegArr;
     const bool full = L > kSegArr && rpad >= over;
     const int nclr = full ? min(pad, over) : 0;
+    // An iteration reads and writes only its own line (reads past it feed discarded steps),
+    // so the lines synchronise on their own: by wave order when a line's lanes lie in one
+    // wave (lpl <= 64, a power of two; each field's half of the workgroup is whole waves),
+    // else by workgroup barriers.  The tile IO after the loop spans all lines.
     float sb[kSegArr];
     for (int it = 0; it < niter; ++it) {
         const bool last = it + 1 == niter;
@@ -1010,7 +1023,7 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
             if (last || grp) slide1<SEG, true>(ln, s0, T, alpha, last ? P.slast : P.sgscale, sb);
             else slide1<SEG, false>(ln, s0, T, alpha, 1.0, sb);
         }
-        __syncthreads();
+        line_sync(lpl);
         // an opaque copy of the lane's element count keeps the compiler from hoisting the
         // kSegArr store predicates of the partial case out of the loop (into spilled SGPRs)
         int nst = myseg;
@@ -1018,8 +1031,9 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
         if (act) stage_store(ln, s0, nst, full, sb);
         if (line_ok)
             for (int e = sid; e < nclr; e += lpl) ln[L + e] = 0.f;
-        __syncthreads();
+        line_sync(lpl);
     }
+    __syncthreads();
 
     if (final_pass) {
         // masked quotient + offset (interpolation.py:367, :428-430), formed as the v lines are stored
@@ -1112,15 +1126,6 @@ __device__ __forceinline__ void line_scan(double t, int sid, int lpl, double *wt
         base = ex;
         total = __shfl(inc, w - 1, w);
     }
-}
-
-// Orders the LDS accesses of one phase of the prefix pass before the next: a workgroup
-// barrier when a line spans several waves; a line within one wave needs only program order
-// (a wave's LDS instructions execute in order), kept by a compiler fence.
-__device__ __forceinline__ void line_sync(int lpl)
-{
-    if (lpl > 64) __syncthreads();
-    else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
 
 // One iteration over the lane's segment: pl[j] = P_{s+j-T-1}, pr[j] = P_{s+j+T+1} for
