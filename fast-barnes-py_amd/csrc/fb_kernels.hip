@@ -962,11 +962,18 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     const int64_t line_in_field = INJECT ? wk.base / F.size[0] + my_line : 0;
 
     if (INJECT) {
-        float4 *z = reinterpret_cast<float4 *>(smem);
-        const int tot = 2 * lines_cap * lstride;
-        const int nz = tot >> 2;
-        for (int i = tid; i < nz; i += nthr) z[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int i = (nz << 2) + tid; i < tot; i += nthr) sv[i] = 0.f;
+        // each line's lanes zero the line's own storage (padding included); lines past nl
+        // and the reserve after the w lines feed only discarded steps
+        if (line_ok) {
+            float *z = mine + (size_t)my_line * lstride;
+            const int head = (int)((16 - ((uintptr_t)z & 15u)) & 15u) >> 2;   // floats before a 16-byte boundary
+            const int nq = (lstride - head) >> 2;
+            float4 *z4 = reinterpret_cast<float4 *>(z + head);
+            for (int i = sid; i < head; i += lpl) z[i] = 0.f;
+            for (int i = sid; i < nq; i += lpl) z4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int i = head + 4 * nq + sid; i < lstride; i += lpl) z[i] = 0.f;
+        }
+        line_sync(lpl);
     } else {
         const int npad = pad + rpad;
         for (int i = tid; i < lines_cap * npad; i += nthr) {
@@ -985,13 +992,14 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
             tile_copy<true>(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
             tile_copy<true>(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
         }
+        __syncthreads();
     }
-    __syncthreads();
 
     if (INJECT) {
+        // a line's injection writes cells 0..L-1 of that line only
         inject_line(F, line_in_field, injector, iw, sid, fsel, off, vals, line_start, line_count, pts, val,
                     mine + my_line * lstride + pad);
-        __syncthreads();
+        line_sync(lpl);
     }
 
     // n-fold tailed box filter, in place.  Every lane owns kSegArr consecutive elements;
