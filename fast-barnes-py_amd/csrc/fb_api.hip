@@ -796,7 +796,7 @@ int plan_run(fb_plan *P, const double *pts, const double *val, float *out, float
     uint32_t *vs = P->d_vals2;
     if (local) {
         if ((rc = launch_field_sort(P->d_fields, P->d_pt_prefix, P->nfields, (int)P->max_n, P->d_keys,
-                                    P->d_line_start, P->d_vals2, s)))
+                                    P->d_line_start, P->d_line_count, P->d_vals2, s)))
             return rc;
     } else {
         // d_vals holds the slots and d_keys2 the rows; both, with d_keys, are free after the

@@ -92,29 +92,90 @@ __device__ __forceinline__ void block_bitonic(unsigned long long *sk, int np2)
     __syncthreads();
 }
 
-// One workgroup per field: the field's (cell key, point index) pairs sorted in LDS by a
-// bitonic network on the 64-bit composite (equal keys stay in point order, as in the
-// row-bucket sort), and the field's points inside the grid written in that order from
-// its first row start on.  Fields of at most kLocalSortMax points.
+// One workgroup per field: the field's points inside the grid written in (cell key, point
+// index) order from its first row start on (equal keys stay in point order, as in the row-bucket
+// sort).  Keys order by row first (key = cell_off + row*W + x), and k_point_keys has counted each
+// row's points, whose exclusive scan gives each row's start.  When the field has at most
+// kFieldRows rows and none holds more than kRankMax points, every point goes to its row's
+// start plus its rank among the row's points: the points are dropped into their rows' ranges
+// in LDS (atomic cursors, any order), then each counts the row's entries below its own
+// composite.  Otherwise a bitonic network sorts the 64-bit composites in LDS.  Both give the
+// same order.  Fields of at most kLocalSortMax points.
+constexpr int kFieldRows = 4096;                                    // LDS row cursors
+constexpr int kRankMax = 64;                                        // largest row placed by rank
+constexpr int kSortPerThread = kLocalSortMax / kLocalSortThreads;   // points per thread
+
 __global__ __launch_bounds__(kLocalSortThreads) void k_field_sort(const FieldDev *__restrict__ fields,
                                                                   const int64_t *__restrict__ pt_prefix,
                                                                   const uint32_t *__restrict__ keys,
                                                                   const uint32_t *__restrict__ row_start,
+                                                                  const uint32_t *__restrict__ row_count,
                                                                   uint32_t *__restrict__ out_vals)
 {
     extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ uint32_t cur[kFieldRows];
+    __shared__ uint32_t wmax[kLocalSortThreads / 64];
     unsigned long long *sk = reinterpret_cast<unsigned long long *>(smem);
     const int f = blockIdx.x;
     const int tid = threadIdx.x;
+    const FieldDev &F = fields[f];
     const int64_t p0 = pt_prefix[f];
     const int n = (int)(pt_prefix[f + 1] - p0);
+    const int64_t r0 = F.xline_off;
+    const int64_t nr = F.size[1] * F.size[2];   // x-lines (rows) of the field
+    const uint32_t dst = row_start[r0];
+
+    // the field's largest row
+    uint32_t m = 0;
+    if (nr <= kFieldRows)
+        for (int r = tid; r < nr; r += kLocalSortThreads) m = max(m, row_count[r0 + r]);
+    for (int d = 32; d >= 1; d >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, d));
+    if ((tid & 63) == 0) wmax[tid >> 6] = m;
+    __syncthreads();
+    uint32_t mx = 0;
+#pragma unroll
+    for (int w = 0; w < kLocalSortThreads / 64; ++w) mx = max(mx, wmax[w]);
+
+    if (nr <= kFieldRows && mx <= (uint32_t)kRankMax) {
+        for (int r = tid; r < nr; r += kLocalSortThreads) cur[r] = 0u;
+        __syncthreads();
+        const uint32_t cbase = (uint32_t)F.cell_off, W = (uint32_t)F.size[0];
+        unsigned long long ev[kSortPerThread];
+        uint32_t rs[kSortPerThread], rc[kSortPerThread];
+#pragma unroll
+        for (int k = 0; k < kSortPerThread; ++k) {
+            const int i = tid + k * kLocalSortThreads;
+            rc[k] = 0u;
+            rs[k] = 0u;
+            ev[k] = 0ull;
+            if (i < n) {
+                const uint32_t key = keys[p0 + i];
+                if (key != kInvalidKey) {
+                    const uint32_t r = (key - cbase) / W;
+                    rs[k] = row_start[r0 + r] - dst;   // the row's range in the field's list
+                    rc[k] = row_count[r0 + r];
+                    ev[k] = ((unsigned long long)key << 32) | (unsigned long long)(uint32_t)(p0 + i);
+                    sk[rs[k] + atomicAdd(&cur[r], 1u)] = ev[k];
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kSortPerThread; ++k) {
+            if (rc[k] == 0u) continue;
+            uint32_t rank = 0;
+            for (uint32_t j = 0; j < rc[k]; ++j) rank += sk[rs[k] + j] < ev[k] ? 1u : 0u;
+            out_vals[dst + rs[k] + rank] = (uint32_t)ev[k];
+        }
+        return;
+    }
+
     int np2 = 1;
     while (np2 < n) np2 <<= 1;
     for (int i = tid; i < np2; i += kLocalSortThreads)
         sk[i] = i < n ? ((unsigned long long)keys[p0 + i] << 32) | (unsigned long long)(uint32_t)(p0 + i) : ~0ull;
     __syncthreads();
     block_bitonic(sk, np2);
-    const uint32_t dst = row_start[fields[f].xline_off];
     for (int i = tid; i < n; i += kLocalSortThreads) {
         const unsigned long long e = sk[i];
         if ((uint32_t)(e >> 32) != kInvalidKey) out_vals[dst + i] = (uint32_t)e;
@@ -382,7 +443,8 @@ int field_sort_init()
 }
 
 int launch_field_sort(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int max_n,
-                      const uint32_t *d_keys, const uint32_t *d_row_start, uint32_t *d_out_vals, hipStream_t stream)
+                      const uint32_t *d_keys, const uint32_t *d_row_start, const uint32_t *d_row_count,
+                      uint32_t *d_out_vals, hipStream_t stream)
 {
     if (nfields <= 0) return FB_OK;
     if (max_n > kLocalSortMax) {
@@ -392,7 +454,7 @@ int launch_field_sort(const FieldDev *d_fields, const int64_t *d_pt_prefix, int 
     int np2 = 1;
     while (np2 < max_n) np2 <<= 1;
     hipLaunchKernelGGL(k_field_sort, dim3(nfields), dim3(kLocalSortThreads), (size_t)np2 * 8, stream, d_fields,
-                       d_pt_prefix, d_keys, d_row_start, d_out_vals);
+                       d_pt_prefix, d_keys, d_row_start, d_row_count, d_out_vals);
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }
