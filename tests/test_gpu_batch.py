@@ -319,6 +319,54 @@ def test_row_buckets_with_crowded_rows():
                 assert float(np.sqrt(np.mean(d * d))) <= 1e-5 * float(np.ptp(c['val']))
 
 
+def test_field_sort_rank_and_bitonic_paths():
+    """Fields of at most 8192 points are sorted in LDS: by rank within their rows when no row
+    holds more than 64 points and the field has at most 4096 rows, else by a bitonic network.
+    Fields at both sides of each limit: a row of exactly 64 points and one of 65 (with
+    duplicate cells, so the point-index order of equal keys matters), 4096 and 4097 rows, and a
+    1D field of 64 and 65 points (one row).  The FP64 results must equal the oracle's bit for
+    bit (the per-cell point order)."""
+    from fastbarnes.batched import BarnesBatch
+    from oracle import oracle as orc
+    rng = np.random.default_rng(91)
+    step = 0.25
+    x0 = np.array([-26.0, 34.0])
+    cases = []
+    for crowd in (64, 65):
+        W, H = 120, 60
+        xs = rng.uniform(x0[0], x0[0] + (W - 1.2) * step, crowd)
+        xs[: crowd // 4] = xs[0]                      # equal cells on the crowded row
+        crowded = np.stack([xs, np.full(crowd, x0[1] + 20.5 * step)], axis=1)
+        sparse = np.stack([rng.uniform(x0[0], x0[0] + (W - 1) * step, 700),
+                           rng.uniform(x0[1], x0[1] + (H - 1) * step, 700)], axis=1)
+        sparse = sparse[np.floor((sparse[:, 1] - x0[1]) / step) != 20]   # the crowded row holds `crowd`
+        p = np.concatenate([crowded, sparse])
+        p = p[rng.permutation(len(p))]
+        cases.append(dict(pts=p, val=np.sin(p[:, 0] / 3.0) * 10 + p[:, 1], sigma=1.1, x0=x0, step=step,
+                          size=(W, H), num_iter=4, max_dist=3.5))
+    for H in (4096, 4097):
+        W = 40
+        p = np.stack([rng.uniform(x0[0], x0[0] + (W - 1) * step, 900),
+                      rng.uniform(x0[1], x0[1] + (H - 1) * step, 900)], axis=1)
+        cases.append(dict(pts=p, val=np.cos(p[:, 1] / 7.0) + p[:, 0], sigma=1.1, x0=x0, step=step, size=(W, H),
+                          num_iter=4, max_dist=3.5))
+    cases1 = []
+    for n in (64, 65):
+        p = rng.uniform(-3.0, 3.0, (n, 1))
+        p[:8, 0] = p[0, 0]
+        cases1.append(dict(pts=p, val=np.sin(p[:, 0]) + rng.normal(0, 0.1, n), sigma=0.4, x0=-3.5, step=0.05,
+                           size=(150,), num_iter=4, max_dist=3.5))
+    for group in (cases, cases1):
+        batch = BarnesBatch([_desc(c) for c in group], precision='fp64')
+        outs = batch.interpolate([c['pts'] for c in group], [c['val'] for c in group])
+        for c, o in zip(group, outs):
+            pts = c['pts'] if len(c['size']) > 1 else c['pts'][:, 0]
+            size = c['size'] if len(c['size']) > 1 else c['size'][0]
+            ref = orc.interpolate_convol(pts, c['val'], c['sigma'], c['x0'], c['step'], size,
+                                         num_iter=c['num_iter'], max_dist=c['max_dist'])
+            _bitwise(o.reshape(ref.shape), ref)
+
+
 def test_choose_pass_rejects_bad_choices():
     from fastbarnes.batched import BarnesBatch
     c = _fields(3)[0]
