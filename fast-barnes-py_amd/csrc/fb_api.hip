@@ -788,10 +788,17 @@ int plan_run(fb_plan *P, const double *pts, const double *val, float *out, float
     const bool local = P->max_n <= kLocalSortMax;
     if ((rc = launch_zero(P->d_line_count, P->nxlines * 4, s))) return rc;
     if (!local && (rc = launch_zero(P->d_big_count, 8, s))) return rc;
-    if ((rc = launch_offsets(P->d_fields, P->nfields, P->max_n, val, P->d_off_part, P->d_offsets, s))) return rc;
-    if ((rc = launch_point_keys(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints, pts, P->d_keys, P->d_vals,
-                                P->d_line_count, local ? nullptr : P->d_vals, local ? nullptr : P->d_keys2, s)))
-        return rc;
+    if (offsets_single_pass(P->max_n, P->d_off_part)) {
+        if ((rc = launch_point_keys_offsets(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints, pts, P->d_keys,
+                                            P->d_vals, P->d_line_count, local ? nullptr : P->d_vals,
+                                            local ? nullptr : P->d_keys2, val, P->d_offsets, s)))
+            return rc;
+    } else {
+        if ((rc = launch_offsets(P->d_fields, P->nfields, P->max_n, val, P->d_off_part, P->d_offsets, s))) return rc;
+        if ((rc = launch_point_keys(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints, pts, P->d_keys, P->d_vals,
+                                    P->d_line_count, local ? nullptr : P->d_vals, local ? nullptr : P->d_keys2, s)))
+            return rc;
+    }
     if ((rc = scan_exclusive(P->d_line_count, P->d_line_start, P->nxlines, P->d_scan_tmp, s))) return rc;
     uint32_t *vs = P->d_vals2;
     if (local) {

@@ -38,12 +38,12 @@ __device__ __forceinline__ int find_seg(const int64_t *__restrict__ prefix, int 
 // value offsets: (min + max)/2 over all values of a field, NaN if any value is NaN
 // (numpy's amin/amax propagate NaN), interpolation.py:209
 
-__global__ __launch_bounds__(kBlock) void k_offsets(const FieldDev *__restrict__ fields,
-                                                    const double *__restrict__ val, double *__restrict__ offsets)
+__device__ __forceinline__ void offsets_block(const FieldDev *__restrict__ fields, const double *__restrict__ val,
+                                              double *__restrict__ offsets, int f)
 {
     __shared__ double smin[kBlock], smax[kBlock];
     __shared__ int snan;
-    const FieldDev &F = fields[blockIdx.x];
+    const FieldDev &F = fields[f];
     const int tid = threadIdx.x;
     if (tid == 0) snan = 0;
     double mn = INFINITY, mx = -INFINITY;
@@ -65,7 +65,13 @@ __global__ __launch_bounds__(kBlock) void k_offsets(const FieldDev *__restrict__
         }
         __syncthreads();
     }
-    if (tid == 0) offsets[blockIdx.x] = snan ? __builtin_nan("") : (smin[0] + smax[0]) / 2.0;
+    if (tid == 0) offsets[f] = snan ? __builtin_nan("") : (smin[0] + smax[0]) / 2.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_offsets(const FieldDev *__restrict__ fields,
+                                                    const double *__restrict__ val, double *__restrict__ offsets)
+{
+    offsets_block(fields, val, offsets, blockIdx.x);
 }
 
 // The same over many workgroups per field (large fields): chunk c of field f (blockIdx.x,
@@ -170,17 +176,16 @@ __device__ __forceinline__ double corner_weight(int dim, const PointPos &pp, int
 // lower corner), kInvalidKey for points the injection drops.  The corners of a point lie in
 // the base row (x-line) and the rows above it; counts per base row give, after the scan,
 // each row's range in the sorted pairs.
-__global__ __launch_bounds__(kBlock) void k_point_keys(const FieldDev *__restrict__ fields,
-                                                       const int64_t *__restrict__ pt_prefix, int nfields,
-                                                       int64_t npoints, const double *__restrict__ pts,
-                                                       uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                                                       uint32_t *__restrict__ row_count,
-                                                       uint32_t *__restrict__ slot, uint32_t *__restrict__ rowg)
+__device__ __forceinline__ void point_keys_block(const FieldDev *__restrict__ fields,
+                                                 const int64_t *__restrict__ pt_prefix, int nfields, int64_t npoints,
+                                                 const double *__restrict__ pts, uint32_t *__restrict__ keys,
+                                                 uint32_t *__restrict__ vals, uint32_t *__restrict__ row_count,
+                                                 uint32_t *__restrict__ slot, uint32_t *__restrict__ rowg, int64_t blk)
 {
     // row_count (may be null): per-row point counts (integer atomics).  slot/rowg (row buckets): the point's
     // position among its row's points as the atomics handed them out, and its global row
     // (kInvalidKey outside the grid); vals (point indices) is then not written.
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t i = blk * kBlock + threadIdx.x;
     if (i >= npoints) return;
     const int f = find_seg(pt_prefix, nfields, i);
     const FieldDev &F = fields[f];
@@ -201,6 +206,35 @@ __global__ __launch_bounds__(kBlock) void k_point_keys(const FieldDev *__restric
             rowg[i] = (uint32_t)(F.xline_off + row);
         }
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_point_keys(const FieldDev *__restrict__ fields,
+                                                       const int64_t *__restrict__ pt_prefix, int nfields,
+                                                       int64_t npoints, const double *__restrict__ pts,
+                                                       uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                       uint32_t *__restrict__ row_count,
+                                                       uint32_t *__restrict__ slot, uint32_t *__restrict__ rowg)
+{
+    point_keys_block(fields, pt_prefix, nfields, npoints, pts, keys, vals, row_count, slot, rowg, blockIdx.x);
+}
+
+// k_point_keys and k_offsets in one launch (they are independent): the first nkb workgroups
+// take the points, the next nfields the fields' offsets
+__global__ __launch_bounds__(kBlock) void k_point_keys_offsets(const FieldDev *__restrict__ fields,
+                                                               const int64_t *__restrict__ pt_prefix, int nfields,
+                                                               int64_t npoints, const double *__restrict__ pts,
+                                                               uint32_t *__restrict__ keys,
+                                                               uint32_t *__restrict__ vals,
+                                                               uint32_t *__restrict__ row_count,
+                                                               uint32_t *__restrict__ slot,
+                                                               uint32_t *__restrict__ rowg, int64_t nkb,
+                                                               const double *__restrict__ val,
+                                                               double *__restrict__ offsets)
+{
+    if ((int64_t)blockIdx.x < nkb)
+        point_keys_block(fields, pt_prefix, nfields, npoints, pts, keys, vals, row_count, slot, rowg, blockIdx.x);
+    else
+        offsets_block(fields, val, offsets, (int)(blockIdx.x - nkb));
 }
 
 // The sub-streams of x-line (y, z): for q < 2*nrows, the points of base row (y - dy, z - dz)
@@ -1507,7 +1541,7 @@ int64_t offsets_part_elems(int nfields, int64_t max_n)
 int launch_offsets(const FieldDev *d_fields, int nfields, int64_t max_n, const double *d_val, double *d_part,
                    double *d_offsets, hipStream_t s)
 {
-    if (!d_part || max_n <= kOffChunk) {
+    if (offsets_single_pass(max_n, d_part)) {
         hipLaunchKernelGGL(k_offsets, dim3(nfields), dim3(kBlock), 0, s, d_fields, d_val, d_offsets);
     } else {
         const int nchunk = (int)((max_n + kOffChunk - 1) / kOffChunk);
@@ -1516,6 +1550,22 @@ int launch_offsets(const FieldDev *d_fields, int nfields, int64_t max_n, const d
         hipLaunchKernelGGL(k_offsets_final, dim3((nfields + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
                            (const double *)d_part, nfields, nchunk, d_offsets);
     }
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
+bool offsets_single_pass(int64_t max_n, const double *d_part) { return !d_part || max_n <= kOffChunk; }
+
+int launch_point_keys_offsets(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
+                              const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_row_count,
+                              uint32_t *d_slot, uint32_t *d_rowg, const double *d_val, double *d_offsets,
+                              hipStream_t s)
+{
+    if (nfields <= 0) return FB_OK;
+    const int64_t nkb = npoints > 0 ? nblocks(npoints) : 0;
+    hipLaunchKernelGGL(k_point_keys_offsets, dim3((unsigned)(nkb + nfields)), dim3(kBlock), 0, s, d_fields,
+                       d_pt_prefix, nfields, npoints, d_pts, d_keys, d_vals, d_row_count, d_slot, d_rowg, nkb, d_val,
+                       d_offsets);
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }

@@ -29,6 +29,13 @@ int launch_offsets(const FieldDev *d_fields, int nfields, int64_t max_n, const d
 int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
                       const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_row_count,
                       uint32_t *d_slot, uint32_t *d_rowg, hipStream_t stream);
+// launch_point_keys and, when the offsets take one workgroup per field (offsets_single_pass),
+// launch_offsets in one launch
+bool offsets_single_pass(int64_t max_n, const double *d_part);
+int launch_point_keys_offsets(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
+                              const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_row_count,
+                              uint32_t *d_slot, uint32_t *d_rowg, const double *d_val, double *d_offsets,
+                              hipStream_t stream);
 // fb_sort.hip -- row buckets for fields of more than kLocalSortMax points: scatter the in-grid
 // points into their rows (d_row_start from the scan of the counts), then sort each row by
 // (key, point index); d_svals/d_skeys receive the sorted list, d_tkeys/d_tvals are scratch for
