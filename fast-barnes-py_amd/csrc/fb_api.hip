@@ -799,13 +799,15 @@ int plan_run(fb_plan *P, const double *pts, const double *val, float *out, float
                                     P->d_line_count, local ? nullptr : P->d_vals, local ? nullptr : P->d_keys2, s)))
             return rc;
     }
-    if ((rc = scan_exclusive(P->d_line_count, P->d_line_start, P->nxlines, P->d_scan_tmp, s))) return rc;
     uint32_t *vs = P->d_vals2;
     if (local) {
+        // the field sort scans each field's row counts itself (a field's list starts at its
+        // first point index)
         if ((rc = launch_field_sort(P->d_fields, P->d_pt_prefix, P->nfields, (int)P->max_n, P->d_keys,
-                                    P->d_line_start, P->d_line_count, P->d_vals2, s)))
+                                    P->d_line_count, P->d_line_start, P->d_vals2, s)))
             return rc;
     } else {
+        if ((rc = scan_exclusive(P->d_line_count, P->d_line_start, P->nxlines, P->d_scan_tmp, s))) return rc;
         // d_vals holds the slots and d_keys2 the rows; both, with d_keys, are free after the
         // scatter and serve as scratch for rows of more than kLocalSortMax points
         if ((rc = row_bucket_sort(P->d_keys, P->d_vals, P->d_keys2, P->npoints, P->d_line_start, P->d_line_count,

@@ -10,11 +10,12 @@ namespace fb {
 int64_t scan_temp_elems(int64_t n);
 int scan_exclusive(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tmp, hipStream_t stream);
 // per-field sort in LDS (fields of at most kLocalSortMax points): the sorted in-grid points
-// of field f from out_vals[row_start[first x-line of f]] on; the same order as the row buckets.
-// d_row_count/d_row_start: the per-row point counts and their exclusive scan
+// of field f from out_vals[pt_prefix[f]] on, in the same order as the row buckets; from the
+// per-row point counts d_row_count it writes the rows' starts d_row_start (the range of field
+// f begins at pt_prefix[f], so no scan over all rows is needed)
 int field_sort_init();
 int launch_field_sort(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int max_n,
-                      const uint32_t *d_keys, const uint32_t *d_row_start, const uint32_t *d_row_count,
+                      const uint32_t *d_keys, const uint32_t *d_row_count, uint32_t *d_row_start,
                       uint32_t *d_out_vals, hipStream_t stream);
 
 // fb_kernels.hip -- injection front end

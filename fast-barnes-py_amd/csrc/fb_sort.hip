@@ -93,48 +93,77 @@ __device__ __forceinline__ void block_bitonic(unsigned long long *sk, int np2)
 }
 
 // One workgroup per field: the field's points inside the grid written in (cell key, point
-// index) order from its first row start on (equal keys stay in point order, as in the row-bucket
-// sort).  Keys order by row first (key = cell_off + row*W + x), and k_point_keys has counted each
-// row's points, whose exclusive scan gives each row's start.  When the field has at most
-// kFieldRows rows and none holds more than kRankMax points, every point goes to its row's
-// start plus its rank among the row's points: the points are dropped into their rows' ranges
-// in LDS (atomic cursors, any order), then each counts the row's entries below its own
-// composite.  Otherwise a bitonic network sorts the 64-bit composites in LDS.  Both give the
-// same order.  Fields of at most kLocalSortMax points.
-constexpr int kFieldRows = 4096;                                    // LDS row cursors
+// index) order (equal keys stay in point order, as in the row-bucket sort) into the field's
+// range of the list, which begins at the field's first point index (pt_prefix[f]; the range
+// holds n slots, of which the in-grid points fill the first).  Keys order by row first (key =
+// cell_off + row*W + x), and k_point_keys has counted each row's points: the workgroup scans the
+// field's row counts into the rows' starts (written to row_start, absolute).  When the field
+// has at most kFieldRows rows and none holds more than kRankMax points, every point goes to
+// its row's start plus its rank among the row's points: the points are dropped into their
+// rows' ranges in LDS (atomic cursors, any order), then each counts the row's entries below
+// its own composite.  Otherwise a bitonic network sorts the 64-bit composites in LDS.  Both
+// give the same order.  Fields of at most kLocalSortMax points.
+constexpr int kFieldRows = 4096;                                    // LDS row starts and cursors
 constexpr int kRankMax = 64;                                        // largest row placed by rank
 constexpr int kSortPerThread = kLocalSortMax / kLocalSortThreads;   // points per thread
+constexpr int kSortWaves = kLocalSortThreads / 64;
 
 __global__ __launch_bounds__(kLocalSortThreads) void k_field_sort(const FieldDev *__restrict__ fields,
                                                                   const int64_t *__restrict__ pt_prefix,
                                                                   const uint32_t *__restrict__ keys,
-                                                                  const uint32_t *__restrict__ row_start,
                                                                   const uint32_t *__restrict__ row_count,
+                                                                  uint32_t *__restrict__ row_start,
                                                                   uint32_t *__restrict__ out_vals)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ uint32_t cur[kFieldRows];
-    __shared__ uint32_t wmax[kLocalSortThreads / 64];
+    __shared__ uint32_t rst[kFieldRows];
+    __shared__ uint32_t wsum[kSortWaves];
     unsigned long long *sk = reinterpret_cast<unsigned long long *>(smem);
     const int f = blockIdx.x;
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
     const FieldDev &F = fields[f];
     const int64_t p0 = pt_prefix[f];
     const int n = (int)(pt_prefix[f + 1] - p0);
     const int64_t r0 = F.xline_off;
     const int64_t nr = F.size[1] * F.size[2];   // x-lines (rows) of the field
-    const uint32_t dst = row_start[r0];
+    const uint32_t dst = (uint32_t)p0;
 
-    // the field's largest row
-    uint32_t m = 0;
-    if (nr <= kFieldRows)
-        for (int r = tid; r < nr; r += kLocalSortThreads) m = max(m, row_count[r0 + r]);
+    // exclusive scan of the row counts (chunks of one row per thread) and the largest row
+    uint32_t carry = 0, m = 0;
+    for (int64_t c0 = 0; c0 < nr; c0 += kLocalSortThreads) {
+        const int64_t r = c0 + tid;
+        const uint32_t v = r < nr ? row_count[r0 + r] : 0u;
+        m = max(m, v);
+        uint32_t inc = v;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(inc, d);
+            if (lane >= d) inc += o;
+        }
+        if (lane == 63) wsum[wv] = inc;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) {
+            const uint32_t x = wsum[w];
+            before += w < wv ? x : 0u;
+            total += x;
+        }
+        const uint32_t ex = carry + before + inc - v;
+        if (r < nr) {
+            row_start[r0 + r] = dst + ex;
+            if (r < kFieldRows) rst[r] = ex;
+        }
+        carry += total;
+        __syncthreads();   // wsum is rewritten by the next chunk
+    }
     for (int d = 32; d >= 1; d >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, d));
-    if ((tid & 63) == 0) wmax[tid >> 6] = m;
+    if (lane == 0) wsum[wv] = m;
     __syncthreads();
     uint32_t mx = 0;
 #pragma unroll
-    for (int w = 0; w < kLocalSortThreads / 64; ++w) mx = max(mx, wmax[w]);
+    for (int w = 0; w < kSortWaves; ++w) mx = max(mx, wsum[w]);
 
     if (nr <= kFieldRows && mx <= (uint32_t)kRankMax) {
         for (int r = tid; r < nr; r += kLocalSortThreads) cur[r] = 0u;
@@ -152,7 +181,7 @@ __global__ __launch_bounds__(kLocalSortThreads) void k_field_sort(const FieldDev
                 const uint32_t key = keys[p0 + i];
                 if (key != kInvalidKey) {
                     const uint32_t r = (key - cbase) / W;
-                    rs[k] = row_start[r0 + r] - dst;   // the row's range in the field's list
+                    rs[k] = rst[r];   // the row's range in the field's list
                     rc[k] = row_count[r0 + r];
                     ev[k] = ((unsigned long long)key << 32) | (unsigned long long)(uint32_t)(p0 + i);
                     sk[rs[k] + atomicAdd(&cur[r], 1u)] = ev[k];
@@ -443,7 +472,7 @@ int field_sort_init()
 }
 
 int launch_field_sort(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int max_n,
-                      const uint32_t *d_keys, const uint32_t *d_row_start, const uint32_t *d_row_count,
+                      const uint32_t *d_keys, const uint32_t *d_row_count, uint32_t *d_row_start,
                       uint32_t *d_out_vals, hipStream_t stream)
 {
     if (nfields <= 0) return FB_OK;
@@ -454,7 +483,7 @@ int launch_field_sort(const FieldDev *d_fields, const int64_t *d_pt_prefix, int 
     int np2 = 1;
     while (np2 < max_n) np2 <<= 1;
     hipLaunchKernelGGL(k_field_sort, dim3(nfields), dim3(kLocalSortThreads), (size_t)np2 * 8, stream, d_fields,
-                       d_pt_prefix, d_keys, d_row_start, d_row_count, d_out_vals);
+                       d_pt_prefix, d_keys, d_row_count, d_row_start, d_out_vals);
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }
