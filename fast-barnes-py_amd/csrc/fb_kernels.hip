@@ -1010,11 +1010,23 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
         line_sync(lpl);
     } else {
         const int npad = pad + rpad;
-        for (int i = tid; i < lines_cap * npad; i += nthr) {
-            const int l = i / npad, r = i - l * npad;
-            const int o = l * lstride + (r < pad ? r : L + r);
-            sv[o] = 0.f;
-            sw[o] = 0.f;
+        if (npad <= 64) {
+            // wave w zeroes the pads of lines w, w + waves, ... (lane = pad position)
+            const int lane = tid & 63;
+            if (lane < npad) {
+                const int o0 = lane < pad ? lane : L + lane;
+                for (int l = tid >> 6; l < lines_cap; l += nthr >> 6) {
+                    sv[l * lstride + o0] = 0.f;
+                    sw[l * lstride + o0] = 0.f;
+                }
+            }
+        } else {
+            for (int i = tid; i < lines_cap * npad; i += nthr) {
+                const int l = i / npad, r = i - l * npad;
+                const int o = l * lstride + (r < pad ? r : L + r);
+                sv[o] = 0.f;
+                sw[o] = 0.f;
+            }
         }
         if (adj_g)
             adjoint_load(out + F.out_off + wk.base, wsave + F.out_off + wk.base, adj_g + F.out_off + wk.base, off,
