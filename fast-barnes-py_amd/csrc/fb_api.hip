@@ -37,20 +37,8 @@ int fail(int code, const std::string &msg)
 
 // ---- reference parameter formulas ------------------------------------------
 
-// float ** int as the reference evaluates it under Numba (binary exponentiation)
-double ipow(double a, int64_t e)
-{
-    double r = 1.0;
-    const bool inv = e < 0;
-    if (inv) e = -e;
-    while (e != 0) {
-        if (e & 1) r *= a;
-        e >>= 1;
-        a *= a;
-    }
-    return inv ? 1.0 / r : r;
-}
-
+// int ** int with 64-bit wrap-around: the plain convolution's kernel_size**num_iter, an
+// integer array to an integer power (interpolation.py:669)
 int64_t ipow_int(int64_t a, int64_t e)
 {
     uint64_t r = 1, b = (uint64_t)a;
@@ -102,13 +90,16 @@ double sigma_effective(double sigma, double step, int64_t n)
     return std::sqrt((double)n / 3.0 * (double)T * (double)(T + 1)) * step;
 }
 
-// conv_scale_factor, interpolation.py:424-425 (tailed) and :669-670 (plain)
+// conv_scale_factor, interpolation.py:424-425 (tailed) and :669-670 (plain).  The tailed
+// (kernel_size+2*tail_value)**num_iter is a float64 array to an integer power: NumPy's power
+// ufunc runs its float64 loop on it (the exponent cast to float64), which calls the C
+// library's pow, as here.  The plain kernel_size**num_iter stays an integer power.
 double threshold(int tail, int dim, const double *sigma, const double *step, const int32_t *K, int64_t n,
                  const double *alpha, double mdw)
 {
     double prod = 1.0;
     for (int m = 0; m < dim; ++m) {
-        const double p = tail ? ipow((double)K[m] + 2 * alpha[m], n) : (double)ipow_int(K[m], n);
+        const double p = tail ? std::pow((double)K[m] + 2 * alpha[m], (double)n) : (double)ipow_int(K[m], n);
         prod *= p / std::sqrt(2 * 3.141592653589793) / (sigma[m] / step[m]);
     }
     return prod * mdw;
@@ -442,6 +433,38 @@ struct fb_plan {
     const void *g_pts = nullptr, *g_val = nullptr, *g_out = nullptr;
     hipStream_t g_stream = nullptr;
 
+    // The last use of the plan's buffers on each stream it ran on: the destructor waits for
+    // these before the buffers go back to the pool, where another plan may take them at once.
+    std::vector<std::pair<hipStream_t, hipEvent_t>> uses;
+
+    // Records the plan's work queued so far on s.  Work queued while the caller captures s into
+    // a graph of its own is not recorded: that graph holds the plan's buffers, so the plan must
+    // outlive it.
+    int mark_use(hipStream_t s)
+    {
+        hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+            (void)hipGetLastError();
+            st = hipStreamCaptureStatusNone;
+        }
+        if (st != hipStreamCaptureStatusNone) return FB_OK;
+        for (auto &u : uses)
+            if (u.first == s) {
+                FB_HIP_CHECK(hipEventRecord(u.second, s));
+                return FB_OK;
+            }
+        hipEvent_t e = nullptr;
+        FB_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        if (hipEventRecord(e, s) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipEventDestroy(e);
+            FB_HIP_CHECK(hipStreamSynchronize(s));   // no event: wait here instead
+            return FB_OK;
+        }
+        uses.push_back({s, e});
+        return FB_OK;
+    }
+
     template <class T>
     int alloc(T **p, size_t count)
     {
@@ -464,9 +487,18 @@ struct fb_plan {
     }
     ~fb_plan()
     {
+        // kernels queued on a caller's stream may still read or write the buffers
+        for (auto &u : uses) {
+            (void)hipEventSynchronize(u.second);
+            (void)hipEventDestroy(u.second);
+        }
+        for (hipEvent_t e : events) {
+            (void)hipEventSynchronize(e);
+            (void)hipEventDestroy(e);
+        }
+        (void)hipGetLastError();
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
-        for (hipEvent_t e : events) (void)hipEventDestroy(e);
         for (auto &b : bufs) pool_free(device, b.p, b.bytes);
     }
 };
@@ -858,10 +890,19 @@ int check_device_ptr(const void *p, const char *what)
     return FB_OK;
 }
 
-// scoped device buffers for the single-call entry points
+// scoped device buffers for the single-call entry points; with `stream` set, the destructor
+// waits for the work queued there (an error return may leave kernels in flight) before the
+// buffers go back to the pool
 struct Scratch {
     int device = 0;
+    hipStream_t stream = nullptr;
+    bool has_stream = false;
     std::vector<DevBuf> b;
+    void on(hipStream_t s)
+    {
+        stream = s;
+        has_stream = true;
+    }
     template <class T>
     int get(T **p, size_t count)
     {
@@ -874,6 +915,7 @@ struct Scratch {
     }
     ~Scratch()
     {
+        if (has_stream && !b.empty() && hipStreamSynchronize(stream) != hipSuccess) (void)hipGetLastError();
         for (auto &x : b) pool_free(device, x.p, x.bytes);
     }
 };
@@ -951,7 +993,9 @@ int fb_plan_create(fb_plan **plan, const fb_field *fields, int nfields, int prec
 int fb_plan_execute(fb_plan *P, const double *d_pts, const double *d_val, float *d_out, void *stream)
 {
     if (!P || !d_pts || !d_val || !d_out) return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument");
-    return plan_run(P, d_pts, d_val, d_out, nullptr, (hipStream_t)stream);
+    const int rc = plan_run(P, d_pts, d_val, d_out, nullptr, (hipStream_t)stream);
+    const int rc2 = P->mark_use((hipStream_t)stream);
+    return rc ? rc : rc2;
 }
 
 int fb_plan_execute_graph(fb_plan *P, const double *d_pts, const double *d_val, float *d_out, void *stream)
@@ -976,7 +1020,7 @@ int fb_plan_execute_graph(fb_plan *P, const double *d_pts, const double *d_val, 
         P->g_pts = d_pts; P->g_val = d_val; P->g_out = d_out; P->g_stream = s;
     }
     FB_HIP_CHECK(hipGraphLaunch(P->gexec, s));
-    return FB_OK;
+    return P->mark_use(s);
 }
 
 int fb_plan_forward(fb_plan *P, const double *d_pts, const double *d_val, float *d_out, float *d_wsave, void *stream)
@@ -984,7 +1028,9 @@ int fb_plan_forward(fb_plan *P, const double *d_pts, const double *d_val, float 
     if (!P || !d_pts || !d_val || !d_out || !d_wsave) return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument");
     if (P->precision != FB_PREC_FP32C)
         return fail(FB_ERR_UNSUPPORTED, "the training forward and the adjoint need a plan of precision FP32C");
-    return plan_run(P, d_pts, d_val, d_out, d_wsave, (hipStream_t)stream);
+    const int rc = plan_run(P, d_pts, d_val, d_out, d_wsave, (hipStream_t)stream);
+    const int rc2 = P->mark_use((hipStream_t)stream);
+    return rc ? rc : rc2;
 }
 
 int fb_plan_backward(fb_plan *P, const double *d_pts, const double *d_val, const float *d_out, const float *d_wsave,
@@ -995,16 +1041,17 @@ int fb_plan_backward(fb_plan *P, const double *d_pts, const double *d_val, const
     if (P->precision != FB_PREC_FP32C)
         return fail(FB_ERR_UNSUPPORTED, "the training forward and the adjoint need a plan of precision FP32C");
     hipStream_t s = (hipStream_t)stream;
-    int rc;
-    if ((rc = launch_offsets(P->d_fields, P->nfields, P->max_n, d_val, P->d_off_part, P->d_offsets, s))) return rc;
+    int rc = launch_offsets(P->d_fields, P->nfields, P->max_n, d_val, P->d_off_part, P->d_offsets, s);
     // the same scaled line passes; the first one forms a = g/W and b = -a(f - offset) in its tile
     // load, so a and b never round-trip through HBM
-    for (int a = 0; a < P->dim; ++a)
-        if ((rc = run_pass(P, a, false, nullptr, nullptr, nullptr, a == 0 ? const_cast<float *>(d_out) : nullptr,
-                           a == 0 ? const_cast<float *>(d_wsave) : nullptr, 0, s, a == 0 ? d_gout : nullptr)))
-            return rc;
-    return launch_adj_gather(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints, d_pts, d_val, P->d_offsets,
-                             P->d_v, P->d_w, P->dim >= 2 ? P->d_pass[1] : nullptr, d_gval, d_gpts, s);
+    for (int a = 0; a < P->dim && !rc; ++a)
+        rc = run_pass(P, a, false, nullptr, nullptr, nullptr, a == 0 ? const_cast<float *>(d_out) : nullptr,
+                      a == 0 ? const_cast<float *>(d_wsave) : nullptr, 0, s, a == 0 ? d_gout : nullptr);
+    if (!rc)
+        rc = launch_adj_gather(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints, d_pts, d_val, P->d_offsets,
+                               P->d_v, P->d_w, P->dim >= 2 ? P->d_pass[1] : nullptr, d_gval, d_gpts, s);
+    const int rc2 = P->mark_use(s);
+    return rc ? rc : rc2;
 }
 
 // Device time (ms, mean of 5 after a warm-up) of the work items of the fields selected by
@@ -1025,6 +1072,7 @@ static int time_pass_subset(fb_plan *P, int a, const std::vector<PassWork> &work
     if (fw.empty()) return FB_OK;
     Scratch sc;
     sc.device = P->device;
+    sc.on(s);
     PassWork *dw;
     PassField *dp;
     int rc;
@@ -1061,7 +1109,8 @@ int fb_plan_autotune(fb_plan *P, const double *d_pts, const double *d_val, float
     if (P->precision != FB_PREC_FP32C) return FB_OK;   // the FP64 path has nothing to tune
     hipStream_t s = (hipStream_t)stream;
     int rc = plan_run(P, d_pts, d_val, d_out, nullptr, s);   // sorted injection lists for the x pass
-    if (rc) return rc;
+    const int rc2 = P->mark_use(s);
+    if (rc || rc2) return rc ? rc : rc2;
     FB_HIP_CHECK(hipStreamSynchronize(s));
     for (int a = 0; a < P->dim; ++a) {
         std::map<int64_t, std::vector<char>> classes;   // line length -> fields with it
@@ -1262,6 +1311,7 @@ int fb_barnes(int method, int precision, int dim, int64_t n, const double *pts, 
     }
     Scratch sc;
     sc.device = device;
+    sc.on(s);
     int rc;
     const double *d_pts = pts, *d_val = val;
     if (on_dev) {
@@ -1336,6 +1386,7 @@ int fb_barnes(int method, int precision, int dim, int64_t n, const double *pts, 
     float *d_out = (float *)out;
     if (!on_dev && (rc = sc.get(&d_out, cells))) { delete P; return rc; }
     rc = plan_run(P, d_pts, d_val, d_out, nullptr, s);
+    if (const int rc2 = P->mark_use(s)) rc = rc ? rc : rc2;
     if (!rc && !on_dev) {
         hipError_t e = hipMemcpyAsync(out, d_out, cells * sizeof(float), hipMemcpyDeviceToHost, s);
         if (e != hipSuccess) rc = fail(FB_ERR_HIP, std::string("copy of the result failed: ") + hipGetErrorString(e));
@@ -1365,6 +1416,7 @@ int fb_barnes_s2(int method, int precision, int64_t n, const double *pts, const 
     hipStream_t s = stream ? (hipStream_t)stream : lib_stream(device);
     Scratch sc;
     sc.device = device;
+    sc.on(s);
     int rc;
     const double *d_pts = pts, *d_val = val;
     if (on_dev) {
@@ -1442,7 +1494,10 @@ int fb_barnes_s2(int method, int precision, int64_t n, const double *pts, const 
         if (resample) rc = sc.get(&d_lam, lam_cells);
         else d_lam = d_out;
     }
-    if (!rc) rc = plan_run(P, d_lpts, d_val, d_lam, nullptr, s);
+    if (!rc) {
+        rc = plan_run(P, d_lpts, d_val, d_lam, nullptr, s);
+        if (const int rc2 = P->mark_use(s)) rc = rc ? rc : rc2;
+    }
     if (!rc && resample) rc = launch_resample(d_lam, lam_size, lam_x0, x0, step, size, proj, d_out, s);
     if (!rc && !on_dev) {
         hipError_t e = hipMemcpyAsync(out, d_out, out_cells * sizeof(float), hipMemcpyDeviceToHost, s);
@@ -1468,6 +1523,7 @@ int fb_accumulate_lines(int tail, int precision, int64_t nlines, int64_t len, in
     hipStream_t s = stream ? (hipStream_t)stream : lib_stream(device);
     Scratch sc;
     sc.device = device;
+    sc.on(s);
     int rc;
     const int64_t cells = nlines * len;
     double *d_io;
@@ -1574,6 +1630,7 @@ int fb_convolve_grid(int tail, int precision, int dim, const int64_t *size, cons
     F.thr_fast_f = float_threshold(thr);
     Scratch sc;
     sc.device = device;
+    sc.on(s);
     int rc;
     FieldDev *d_f;
     double *d_vg, *d_wg;

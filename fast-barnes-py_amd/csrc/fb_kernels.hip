@@ -1061,7 +1061,8 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     // whole-segment stores (when the host widened the right padding to the overflow of the
     // last segment): the segment crossing the line end writes `over` floats into the
     // padding; the line's lanes then re-zero those of them the next iteration reads (within
-    // the first T+1), after the stores in the wave's program order
+    // the first T+1), after the stores: by the wave's program order when the line lies in one
+    // wave, else after a workgroup barrier
     const int over = (kSegArr - L % kSegArr) % kSegArr;
     const bool full = L > kSegArr && rpad >= over;
     const int nclr = full ? min(pad, over) : 0;
@@ -1083,6 +1084,11 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
         int nst = myseg;
         asm volatile("" : "+v"(nst));
         if (act) stage_store(ln, s0, nst, full, sb);
+        // the lane whose segment crosses the line end stores into the padding; in a line of
+        // several waves it is not in the wave of the lanes that re-zero the padding, so a
+        // barrier orders its stores before theirs (nclr and lpl are the same for the whole
+        // workgroup)
+        if (lpl > 64 && nclr > 0) __syncthreads();
         if (line_ok)
             for (int e = sid; e < nclr; e += lpl) ln[L + e] = 0.f;
         line_sync(lpl);

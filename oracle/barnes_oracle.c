@@ -26,32 +26,19 @@
  *   orc_interpolate_naive        _interpolate_naive           :861-938
  *   orc_interpolate_radius_bf    _interpolate_radius          :809-855  (brute force instead of kd-tree)
  *
- * Integer powers.  The reference runs under Numba, whose scalar lowering of
- * `float ** int` is binary exponentiation (r = 1; while e: if e&1: r *= a;
- * e >>= 1; a *= a) and of `float ** 2` with a literal exponent is a single
- * product.  orc_ipow() and sq() restate exactly that.  These decide only the
- * last ulp of the NaN-mask threshold and of the tail value; which lowering
- * Numba picks is not pinned by any reference test.
+ * Powers.  The tailed threshold (kernel_size+2*tail_value)**num_iter (:424) is a
+ * float64 array raised to an integer: NumPy's power ufunc runs its float64 loop
+ * there (the exponent cast to float64), which calls the C library's pow();
+ * orc_threshold() calls pow() the same way.  The plain threshold
+ * kernel_size**num_iter (:669) is an integer array to an integer power, which
+ * wraps around in 64 bits (orc_ipow_int).  `x**2` with a literal exponent is a
+ * single product (sq).
  */
 #define _GNU_SOURCE
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
-
-/* float ** int as Numba's int_power_impl computes it (see header). */
-static double orc_ipow(double a, int64_t e)
-{
-    double r = 1.0;
-    int invert = 0;
-    if (e < 0) { invert = 1; e = -e; }
-    while (e != 0) {
-        if (e & 1) r *= a;
-        e >>= 1;
-        a *= a;
-    }
-    return invert ? 1.0 / r : r;
-}
 
 /* int ** int with 64-bit wrap-around, as the integer power of the reference. */
 static int64_t orc_ipow_int(int64_t a, int64_t e)
@@ -119,7 +106,7 @@ double orc_threshold(int tail, int dim, const double *sigma, const double *step,
     for (int m = 0; m < dim; m++) {
         double p;
         if (tail)
-            p = orc_ipow((double)kernel_size[m] + 2 * tail_value[m], num_iter);
+            p = pow((double)kernel_size[m] + 2 * tail_value[m], (double)num_iter);
         else
             p = (double)orc_ipow_int(kernel_size[m], num_iter);
         double f = p / sqrt(2 * 3.141592653589793) / (sigma[m] / step[m]);
@@ -130,15 +117,18 @@ double orc_threshold(int tail, int dim, const double *sigma, const double *step,
 
 /* ---- values ------------------------------------------------------------ */
 
-/* _normalize_values :204-212; val is modified in place, offset returned */
+/* _normalize_values :204-212; val is modified in place, offset returned.  np.amin and
+ * np.amax propagate NaN: any NaN value gives a NaN offset (and so an all-NaN field). */
 double orc_normalize_values(double *val, int64_t n)
 {
     double mn = val[0], mx = val[0];
+    int has_nan = isnan(val[0]);
     for (int64_t k = 1; k < n; k++) {
+        if (isnan(val[k])) has_nan = 1;
         if (val[k] < mn) mn = val[k];
         if (val[k] > mx) mx = val[k];
     }
-    double offset = (mn + mx) / 2.0;
+    double offset = has_nan ? NAN : (mn + mx) / 2.0;
     for (int64_t k = 0; k < n; k++) val[k] -= offset;
     return offset;
 }

@@ -60,8 +60,11 @@ def test_parameters_match_oracle(orc, seed):
 def test_threshold_matches_oracle(orc):
     import ctypes
     from fastbarnes import _lib
+    rng = np.random.default_rng(77)
+    extra = [([float(rng.uniform(0.5, 2.0))] * 2, [75.0 / W] * 2, int(n)) for W in (600, 1200, 2400)
+             for n in (3, 4, 5, 6) for _ in range(8)]
     for sig, st, n in [([1.0, 1.0], [1 / 32, 1 / 32], 4), ([1.5] * 3, [0.125] * 3, 4), ([0.7], [0.01], 6),
-                       ([2.0, 0.5], [0.1, 0.05], 3)]:
+                       ([2.0, 0.5], [0.1, 0.05], 3)] + extra:
         dim = len(sig)
         sig_a, st_a = np.array(sig), np.array(st)
         for tail in (True, False):

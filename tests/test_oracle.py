@@ -204,6 +204,80 @@ def test_threshold_closed_form(orc):
     assert thr0 == pytest.approx(expect0, rel=1e-14)
 
 
+def _binary_power(a, e):
+    r = 1.0
+    while e:
+        if e & 1:
+            r *= a
+        e >>= 1
+        a *= a
+    return r
+
+
+def _threshold_params():
+    """(sigma, step, n) over the uneven batch's ranges: sigma ~ U[0.5, 2], W in {600, 1200, 2400},
+    step = 75/W, n in 3..6 (744 combinations)."""
+    rng = np.random.default_rng(2024)
+    out = []
+    for W in (600, 1200, 2400):
+        for n in (3, 4, 5, 6):
+            for s in rng.uniform(0.5, 2.0, 62):
+                out.append((float(s), 75.0 / W, n))
+    return out
+
+
+def test_threshold_power_is_libm_pow(orc):
+    """The tailed threshold's (K + 2 alpha)**n (interpolation.py:424) is NumPy's float64 power loop,
+    i.e. the C library's pow.  Three pins over 744 parameter sets of the uneven batch's ranges:
+    * the oracle's per-axis factor p / sqrt(2 pi) / (sigma/step) * max_dist_weight is the one formed
+      from pow(K + 2 alpha, n) bit for bit;
+    * that pow is checked against the exact rational power (fractions), independent of any pow
+      implementation: within 0.52 ulp everywhere, and the correctly rounded value wherever the
+      exact power is more than 1/50 ulp away from a rounding midpoint (glibc's pow is not
+      correctly rounded at near-midpoints);
+    * binary exponentiation, which a restatement might use instead, differs from it on some sets."""
+    from fractions import Fraction
+    params = _threshold_params()
+    assert len(params) >= 700
+    differ = near_mid = 0
+    for s, d, n in params:
+        T = int(orc.half_kernel_size_opt(s, d, n)[0])
+        a = float(orc.tail_value(s, d, n)[0])
+        base = float(2 * T + 1) + 2 * a
+        p = math.pow(base, n)
+        mdw = 0.5   # a power of two, exact
+        expect = p / math.sqrt(2 * math.pi) / (s / d) * mdw
+        assert orc.threshold(True, s, d, 2 * T + 1, n, a, mdw, 1) == expect, (s, d, n)
+        exact = Fraction(base) ** n
+        rounded = float(exact)               # correctly rounded (exact integer division)
+        ulp = Fraction(math.ulp(rounded))
+        err = abs(Fraction(p) - exact) / ulp
+        assert err <= Fraction(52, 100), (s, d, n, float(err))
+        dist_mid = abs(abs(Fraction(rounded) - exact) / ulp - Fraction(1, 2))
+        if dist_mid > Fraction(1, 50):
+            assert p == rounded, (s, d, n)
+        else:
+            near_mid += 1
+        differ += _binary_power(base, n) != p
+    assert near_mid < len(params) // 20
+    assert differ > 0, 'the parameter set does not separate pow from binary exponentiation'
+
+
+def test_nan_value_propagates_to_offset(orc):
+    """np.amin/np.amax (interpolation.py:209) propagate NaN: a NaN value anywhere makes the
+    offset NaN, and the whole result NaN."""
+    rng = np.random.default_rng(3)
+    pts = rng.uniform(0, 10, (50, 2))
+    for idx in (0, 1, 17, 49):
+        val = rng.normal(size=50)
+        val[idx] = np.nan
+        v = val.copy()
+        _, _, off = orc.inject(pts, v, (0.0, 0.0), 0.25, (40, 40))
+        assert math.isnan(off)
+        out = orc.interpolate_convol(pts, val, 1.0, (0.0, 0.0), 0.25, (40, 40))
+        assert np.isnan(out).all(), idx
+
+
 # ---------------------------------------------------------------------------
 # the reference's Gaussian convergence tables (tests/GaussianApproximationTest.py)
 
