@@ -15,6 +15,11 @@
 
 #include "fb_launch.h"
 
+// Workspace of fb_barnes_noalloc, reserved with the code object on every device the library
+// runs on.  Host code uses the per-device address noalloc_scratch() returns, never this name.
+constexpr size_t kNoAllocBytes = FB_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 namespace fb {
 
 static thread_local std::string g_err;
@@ -465,11 +470,26 @@ struct fb_plan {
         return FB_OK;
     }
 
+    // With `arena` set, the buffers are carved from [arena, arena + arena_cap) in order
+    // (256-byte aligned) instead of the pool, and nothing is allocated or returned.
+    char *arena = nullptr;
+    size_t arena_cap = 0;
+
     template <class T>
     int alloc(T **p, size_t count)
     {
         void *q = nullptr;
         const size_t bytes = count * sizeof(T);
+        if (arena) {
+            const size_t need = round_bytes(bytes ? bytes : 256);
+            if (ws_bytes + need > arena_cap)
+                return fail(FB_ERR_UNSUPPORTED, "the workspace of this call exceeds the fixed scratch array of " +
+                                                    std::to_string(arena_cap) + " bytes of fb_barnes_noalloc; " +
+                                                    "use fb_barnes");
+            *p = (T *)(arena + ws_bytes);
+            ws_bytes += need;
+            return FB_OK;
+        }
         int rc = pool_alloc(device, bytes, &q);
         if (rc) return rc;
         bufs.push_back({q, bytes});
@@ -701,7 +721,10 @@ int run_pass(fb_plan *P, int axis, bool inject, const uint32_t *vals, const doub
     return FB_OK;
 }
 
-int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precision)
+// arena: the plan's device buffers are carved from [arena, arena + arena_cap) instead of the
+// pool (fb_barnes_noalloc)
+int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precision, void *arena = nullptr,
+                size_t arena_cap = 0)
 {
     *out = nullptr;
     if (!fields || nfields < 1) return fail(FB_ERR_INVALID_ARGUMENT, "no fields");
@@ -710,8 +733,12 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
     int rc0 = fast_pass_init();
     if (!rc0) rc0 = field_sort_init();
     if (rc0) return rc0;
+    int device = 0;
+    FB_HIP_CHECK(hipGetDevice(&device));
     fb_plan *P = new fb_plan();
-    FB_HIP_CHECK(hipGetDevice(&P->device));
+    P->device = device;
+    P->arena = (char *)arena;
+    P->arena_cap = arena_cap;
     P->precision = precision;
     P->nfields = nfields;
     P->dim = fields[0].dim;
@@ -887,6 +914,50 @@ int check_device_ptr(const void *p, const char *what)
     }
     if (a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged)
         return fail(FB_ERR_INVALID_ARGUMENT, std::string(what) + " is not device memory");
+    return FB_OK;
+}
+
+// the batch descriptor of the one field of a drop-in call
+fb_field single_field(int method, int dim, int64_t n, const double *sigma, const double *x0, const double *step,
+                      const int64_t *size, int num_iter, double max_dist_weight)
+{
+    fb_field f;
+    std::memset(&f, 0, sizeof(f));
+    f.dim = dim;
+    f.method = method;
+    f.num_iter = num_iter;
+    f.n = n;
+    for (int m = 0; m < 3; ++m) f.size[m] = 1;
+    for (int m = 0; m < dim; ++m) {
+        f.size[m] = size[m];
+        f.sigma[m] = sigma[m];
+        f.x0[m] = x0[m];
+        f.step[m] = step[m];
+    }
+    f.max_dist_weight = max_dist_weight;
+    return f;
+}
+
+// Device address of this device's copy of `scratch`, looked up once per device (with that
+// device current) by its first caller.
+int noalloc_scratch(int device, void **ptr)
+{
+    static std::mutex mu;
+    static std::map<int, void *> addr;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) {
+        *ptr = it->second;
+        return FB_OK;
+    }
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return fail(FB_ERR_HIP, std::string("address of the no-allocation scratch array: ") + hipGetErrorString(e));
+    }
+    addr[device] = p;
+    *ptr = p;
     return FB_OK;
 }
 
@@ -1365,22 +1436,7 @@ int fb_barnes(int method, int precision, int dim, int64_t n, const double *pts, 
     if (method != FB_METHOD_OPTIMIZED_CONVOLUTION && method != FB_METHOD_CONVOLUTION)
         return fail(FB_ERR_INVALID_ARGUMENT, "encountered invalid Barnes interpolation method");
 
-    fb_field f;
-    std::memset(&f, 0, sizeof(f));
-    f.dim = dim;
-    f.method = method;
-    f.num_iter = num_iter;
-    f.n = n;
-    f.pt_offset = 0;
-    f.out_offset = 0;
-    for (int m = 0; m < 3; ++m) f.size[m] = 1;
-    for (int m = 0; m < dim; ++m) {
-        f.size[m] = size[m];
-        f.sigma[m] = sigma[m];
-        f.x0[m] = x0[m];
-        f.step[m] = step[m];
-    }
-    f.max_dist_weight = max_dist_weight;
+    const fb_field f = single_field(method, dim, n, sigma, x0, step, size, num_iter, max_dist_weight);
     fb_plan *P = nullptr;
     if ((rc = plan_create(&P, &f, 1, precision))) return rc;
     float *d_out = (float *)out;
@@ -1394,6 +1450,41 @@ int fb_barnes(int method, int precision, int dim, int64_t n, const double *pts, 
     if (!rc) {
         hipError_t e = hipStreamSynchronize(s);
         if (e != hipSuccess) rc = fail(FB_ERR_HIP, std::string("execution failed: ") + hipGetErrorString(e));
+    }
+    delete P;
+    return rc;
+}
+
+int fb_barnes_noalloc(int method, int precision, int dim, int64_t n, const double *d_pts, const double *d_val,
+                      const double *sigma, const double *x0, const double *step, const int64_t *size, int num_iter,
+                      double max_dist_weight, float *d_out, void *stream)
+{
+    if (dim < 1 || dim > 3)
+        return fail(FB_ERR_INVALID_ARGUMENT, "Barnes interpolation supports only sample points in dimensions 1, 2 or 3");
+    if (n < 1 || !d_pts || !d_val || !sigma || !x0 || !step || !size || !d_out)
+        return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument or no sample points");
+    if (method != FB_METHOD_OPTIMIZED_CONVOLUTION && method != FB_METHOD_CONVOLUTION)
+        return fail(FB_ERR_UNSUPPORTED, "fb_barnes_noalloc runs the convolution methods only");
+    for (int m = 0; m < dim; ++m)
+        if (size[m] < 1) return fail(FB_ERR_INVALID_ARGUMENT, "grid size must be positive");
+    int rc;
+    if ((rc = check_device_ptr(d_pts, "pts")) || (rc = check_device_ptr(d_val, "val")) ||
+        (rc = check_device_ptr(d_out, "out")))
+        return rc;
+    int device = 0;
+    FB_HIP_CHECK(hipGetDevice(&device));
+    void *ws = nullptr;
+    if ((rc = noalloc_scratch(device, &ws))) return rc;
+    const fb_field f = single_field(method, dim, n, sigma, x0, step, size, num_iter, max_dist_weight);
+    fb_plan *P = nullptr;
+    if ((rc = plan_create(&P, &f, 1, precision, ws, kNoAllocBytes))) return rc;
+    const hipStream_t s = (hipStream_t)stream;
+    rc = plan_run(P, d_pts, d_val, d_out, nullptr, s);
+    // wait in any case: kernels queued before an error may still use the scratch array
+    const hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        if (!rc) rc = fail(FB_ERR_HIP, std::string("execution failed: ") + hipGetErrorString(e));
     }
     delete P;
     return rc;

@@ -66,6 +66,8 @@ SIGNATURES = {
                                     _dp]),
     'fb_barnes': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _vp, _vp, _dp, _dp, _dp,
                                  _i64p, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, ctypes.c_uint, _vp]),
+    'fb_barnes_noalloc': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _vp, _vp, _dp, _dp,
+                                         _dp, _i64p, ctypes.c_int, ctypes.c_double, _vp, _vp]),
     'fb_barnes_s2': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, _vp, _vp, _dp, _dp, _dp, _i64p,
                                     ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, ctypes.c_uint, _vp]),
     'fb_accumulate_lines': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,

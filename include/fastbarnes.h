@@ -89,6 +89,22 @@ int fb_barnes(int method, int precision, int dim, int64_t n, const double *pts, 
               int num_iter, double max_dist_weight, double min_weight, void *out, unsigned flags,
               void *stream);
 
+/* fb_barnes for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * The convolution methods only (FB_METHOD_OPTIMIZED_CONVOLUTION, FB_METHOD_CONVOLUTION);
+ * pts, val and out (float32) are device pointers, the other arguments as for fb_barnes.
+ * The workspace is a fixed array of FB_NOALLOC_SCRATCH_BYTES per device, reserved with the
+ * library's code object and held until the process ends: enough for one 2400x1200 field in
+ * either precision; a call whose workspace exceeds it fails with FB_ERR_UNSUPPORTED before
+ * any kernel runs.  Calls no hipMalloc/hipFree.  All calls on one device share that array,
+ * which keeps the last call's contents: calls that overlap in time on one device (other host
+ * threads or streams) race.  stream is a hipStream_t (NULL = the null stream); the call
+ * returns after `out` is written. */
+#define FB_NOALLOC_SCRATCH_BYTES (128ull << 20)
+int fb_barnes_noalloc(int method, int precision, int dim, int64_t n, const double *pts, const double *val,
+                      const double *sigma, const double *x0, const double *step, const int64_t *size,
+                      int num_iter, double max_dist_weight, float *out, void *stream);
+
 /* barnes_S2(pts, val, sigma, x0, step, size, method, num_iter, max_dist, resample)
  *   interpolationS2.py:31-154 (dispatch :146-154):
  *   FB_METHOD_OPTIMIZED_CONVOLUTION = 'optimized_convolution_S2' :157-219 -- Lambert conformal
