@@ -10,8 +10,9 @@ before the timed region; each step replays the batch's captured hipGraph.
 
 Printed JSON line: value = grid points per second of the whole batch; roofline = the
 dominant kernel's algorithmic bytes over its average duration (HIP events on its
-stream); cpu_baseline = the CPU oracle (FP64 C restatement of the reference path,
-one core) on a bounded sample of the same fields.  'workloads' adds the other
+stream), with frac_moved at the bytes this build's kernels move (w kept in float32);
+cpu_baseline = the CPU oracle (FP64 C restatement of the reference path, -O3, OpenMP
+over lines) on a bounded sample of the same fields, on all host cores and on one.  'workloads' adds the other
 workloads, each with its throughput, pipeline roofline fraction and error against the
 FP64 path: cfg3 (256 uneven 2D fields), cfg4 (3D 512x512x128) and cfg4_batch (32 3D
 fields of 256x256x64), cfg5 (16 fields of 2400x1200, N=100000, forward + adjoint), real
@@ -42,6 +43,9 @@ HBM_PEAK_GBS = 8000.0
 #   y-pass: reads v, w (12 B) and writes the float32 result (4 B) per grid point
 BYTES_PER_POINT = {'pass_x': 12.0, 'pass_y': 16.0}
 BYTES_PER_SAMPLE = {'pass_x': 24.0, 'pass_y': 0.0}
+# the bytes this build's passes move: w stays float32 between the passes, so the x pass writes
+# 8 B per grid point and the y pass reads 8 and writes 4
+MOVED_PER_POINT = {'pass_x': 8.0, 'pass_y': 12.0}
 
 
 def log(msg):
@@ -56,24 +60,60 @@ def make_fields(nfields, seed0=100):
     return cfgs, descs
 
 
-def cpu_baseline(cfgs, seconds):
-    """The oracle (FP64 C restatement of the reference, one core) on a bounded sample of the
-    batch's fields: as many whole fields as fit in `seconds` (at least one)."""
+def _cpu_model():
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def _host_threads():
+    """Cores this process may use: its CPU affinity, capped by OMP_NUM_THREADS when set."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get('OMP_NUM_THREADS', '')
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(n, 1)
+
+
+def _time_oracle(cfgs, seconds, threads):
+    """Whole fields of the batch through the oracle's OpenMP build on `threads` threads, one after
+    another until `seconds` have passed (at least one field, at most the whole batch)."""
     from oracle import oracle
-    oracle.lib()
-    done = 0
-    pts_total = 0
+    done = pts_total = 0
     t0 = time.perf_counter()
     for c in cfgs:
         oracle.interpolate_convol(c['pts'], c['val'], c['sigma'], c['x0'], c['step'], c['size'],
-                                  num_iter=c['num_iter'], max_dist=c['max_dist'])
+                                  num_iter=c['num_iter'], max_dist=c['max_dist'], threads=threads)
         done += 1
         pts_total += c['size'][0] * c['size'][1]
         if time.perf_counter() - t0 >= seconds:
             break
     dt = time.perf_counter() - t0
-    return dict(value=pts_total / dt, unit='grid-pts/s', cores=1, kind='port',
-                sample='%d field(s) of 2400x1200, N=3490, n=4 (FP64 oracle, -O2, 1 thread), %.2f s' % (done, dt))
+    return pts_total / dt, done, dt
+
+
+def cpu_baseline(cfgs, seconds):
+    """The oracle (the FP64 C restatement of the reference path, in its loop and operation order;
+    built at -O3 for x86-64-v3 with OpenMP over the independent lines of each pass) on a bounded
+    sample of the batch's fields: on all host cores (the value) and on one core (the
+    reference's own sequential design)."""
+    from oracle import oracle
+    oracle.omp_lib()
+    nthr = _host_threads()
+    flags = 'gcc -O3 -march=x86-64-v3 -fopenmp -ffp-contract=off'
+    one, n1, t1 = _time_oracle(cfgs, seconds / 2, 1)
+    allc, na, ta = _time_oracle(cfgs, seconds / 2, nthr)
+    model = _cpu_model()
+    return dict(value=round(allc, 1), unit='grid-pts/s', cores=nthr, kind='port', cpu=model, flags=flags,
+                sample='%d field(s) of 2400x1200, N=3490, n=4 (FP64 oracle, OpenMP over lines, %d threads), %.2f s'
+                       % (na, nthr, ta),
+                single_core=dict(value=round(one, 1), cores=1,
+                                 sample='%d field(s), 1 thread, %.2f s' % (n1, t1)))
 
 
 # algorithmic bytes per grid point / per sample point of the other workloads' pipelines
@@ -425,19 +465,30 @@ def main():
     kern = max(BYTES_PER_POINT, key=lambda k: stage_ms.get(k, 0.0))
     alg_bytes = BYTES_PER_POINT[kern] * npts + BYTES_PER_SAMPLE[kern] * nsamples
     achieved = alg_bytes / (stage_ms[kern] * 1e-3) / 1e9
-    traffic = None
+    moved_bytes = MOVED_PER_POINT[kern] * npts + BYTES_PER_SAMPLE[kern] * nsamples
+    # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counters of a profiled run of this
+    # workload (profiles/traffic.json names the commit and command it came from); not measured
+    # in this run
+    traffic = traffic_src = None
     tfile = os.path.join(ROOT, 'profiles', 'traffic.json')
     if os.path.exists(tfile):
         with open(tfile) as f:
-            t = json.load(f).get(kern)
+            tj = json.load(f)
+        t = tj.get(kern)
         if t and t.get('fields') == args.fields:
             traffic = t.get('bytes_per_launch')
+            traffic_src = tj.get('source')
     roof = dict(bound='hbm', kernel=kern, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                alg_bytes_per_launch=alg_bytes, avg_launch_ms=round(stage_ms[kern], 5))
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=traffic_src,
+                alg_bytes_per_launch=alg_bytes, avg_launch_ms=round(stage_ms[kern], 5),
+                moved_bytes_per_launch=moved_bytes,
+                frac_moved=round(moved_bytes / (stage_ms[kern] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
     pipe_bytes = 28.0 * npts + 24.0 * nsamples
+    pipe_moved = 20.0 * npts + 24.0 * nsamples
     pipeline = dict(alg_bytes_per_step=pipe_bytes, achieved=round(pipe_bytes / (ms_step * 1e-3) / 1e9, 1),
                     frac=round(pipe_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    moved_bytes_per_step=pipe_moved,
+                    frac_moved=round(pipe_moved / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                     stage_ms={k: round(v, 5) for k, v in stage_ms.items()})
 
     extras = {}
@@ -491,7 +542,6 @@ def main():
     if not args.no_cpu:
         log('cpu baseline (%.0f s budget)' % args.cpu_seconds)
         cpu = cpu_baseline(cfgs, args.cpu_seconds)
-        cpu['value'] = round(cpu['value'], 1)
 
     res = {
         'metric': METRIC, 'value': round(value, 1), 'unit': 'grid-pts/s', 'n_gpus': 1, 'steps': args.steps,

@@ -36,6 +36,9 @@
  */
 #define _GNU_SOURCE
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -270,15 +273,38 @@ double *orc_accumulate_array(double *in_arr, double *h_arr, int64_t arr_len, int
  * This is the `vg[j,:] = _accumulate_*(vg[j,:].copy(), h_arr, ...)` idiom of
  * :404-418, :440-466, :649-663, :685-711. */
 static void convolve_axis(double *grid, int64_t nlines, const int64_t *bases, int64_t len, int64_t estride,
-                          int64_t rect_len, int64_t num_iter, int tail, double alpha, double *a, double *b)
+                          int64_t rect_len, int64_t num_iter, int tail, double alpha)
 {
-    for (int64_t l = 0; l < nlines; l++) {
-        const int64_t base = bases[l];
-        for (int64_t i = 0; i < len; i++) a[i] = grid[base + i * estride];
-        double *r = tail ? orc_accumulate_tail_array(a, b, len, rect_len, num_iter, alpha)
-                         : orc_accumulate_array(a, b, len, rect_len, num_iter);
-        for (int64_t i = 0; i < len; i++) grid[base + i * estride] = r[i];
+    /* Lines are independent, so an OpenMP build (liboracle_omp.so, the timed CPU baseline)
+     * shares them out over threads, each with its own line buffers; the values do not change. */
+#pragma omp parallel
+    {
+        double *a = (double *)malloc(sizeof(double) * (len > 0 ? len : 1));
+        double *b = (double *)malloc(sizeof(double) * (len > 0 ? len : 1));
+#pragma omp for schedule(static)
+        for (int64_t l = 0; l < nlines; l++) {
+            const int64_t base = bases[l];
+            for (int64_t i = 0; i < len; i++) a[i] = grid[base + i * estride];
+            double *r = tail ? orc_accumulate_tail_array(a, b, len, rect_len, num_iter, alpha)
+                             : orc_accumulate_array(a, b, len, rect_len, num_iter);
+            for (int64_t i = 0; i < len; i++) grid[base + i * estride] = r[i];
+        }
+        free(a);
+        free(b);
     }
+}
+
+/* Threads of the OpenMP build's line loops (no effect on the single-threaded build); returns
+ * the number in use. */
+int orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }
 
 /* _convolve_tail_{1,2,3}d (tail=1) :372-479 and _convolve_{1,2,3}d (tail=0) :617-724.
@@ -289,9 +315,6 @@ void orc_convolve(int tail, int dim, double *vg, double *wg, const double *sigma
                   double max_dist_weight)
 {
     int64_t W = size[0], H = dim > 1 ? size[1] : 1, D = dim > 2 ? size[2] : 1;
-    int64_t maxlen = W > H ? W : H; if (D > maxlen) maxlen = D;
-    double *a = (double *)malloc(sizeof(double) * maxlen);
-    double *b = (double *)malloc(sizeof(double) * maxlen);
     int64_t nmax = W * H > W * D ? W * H : W * D; if (H * D > nmax) nmax = H * D;
     int64_t *bases = (int64_t *)malloc(sizeof(int64_t) * (nmax > 0 ? nmax : 1));
     double alpha[3] = {0, 0, 0};
@@ -305,25 +328,26 @@ void orc_convolve(int tail, int dim, double *vg, double *wg, const double *sigma
         int64_t nl = 0;
         for (int64_t z = 0; z < D; z++) for (int64_t y = 0; y < H; y++) bases[nl++] = (z * H + y) * W;
         for (int f = 0; f < 2; f++)
-            convolve_axis(fields[f], nl, bases, W, 1, kernel_size[0], num_iter, tail, alpha[0], a, b);
+            convolve_axis(fields[f], nl, bases, W, 1, kernel_size[0], num_iter, tail, alpha[0]);
     }
     if (dim >= 2) {   /* y pass: lines (z, x) */
         int64_t nl = 0;
         for (int64_t z = 0; z < D; z++) for (int64_t x = 0; x < W; x++) bases[nl++] = z * H * W + x;
         for (int f = 0; f < 2; f++)
-            convolve_axis(fields[f], nl, bases, H, W, kernel_size[1], num_iter, tail, alpha[1], a, b);
+            convolve_axis(fields[f], nl, bases, H, W, kernel_size[1], num_iter, tail, alpha[1]);
     }
     if (dim >= 3) {   /* z pass: lines (y, x) */
         int64_t nl = 0;
         for (int64_t y = 0; y < H; y++) for (int64_t x = 0; x < W; x++) bases[nl++] = y * W + x;
         for (int f = 0; f < 2; f++)
-            convolve_axis(fields[f], nl, bases, D, W * H, kernel_size[2], num_iter, tail, alpha[2], a, b);
+            convolve_axis(fields[f], nl, bases, D, W * H, kernel_size[2], num_iter, tail, alpha[2]);
     }
     double thr = orc_threshold(tail, dim, sigma, step, kernel_size, num_iter, alpha, max_dist_weight);
     const int64_t total = W * H * D;
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < total; i++)
         if (wg[i] < thr) wg[i] = NAN;
-    free(a); free(b); free(bases);
+    free(bases);
 }
 
 /* _interpolate_opt_convol :328-367 (tail=1) and _interpolate_convol :575-612 (tail=0).
@@ -338,8 +362,10 @@ void orc_interpolate_convol(int tail, int dim, const double *pts, const double *
     double offset = orc_normalize_values(val, n);
     int64_t total = 1;
     for (int m = 0; m < dim; m++) total *= size[m];
-    double *vg = (double *)calloc(total, sizeof(double));
-    double *wg = (double *)calloc(total, sizeof(double));
+    double *vg = (double *)malloc(sizeof(double) * (total > 0 ? total : 1));
+    double *wg = (double *)malloc(sizeof(double) * (total > 0 ? total : 1));
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < total; i++) vg[i] = wg[i] = 0.0;
     orc_inject(dim, vg, wg, pts, val, n, x0, step, size);
     int32_t T[3], K[3];
     double alpha[3] = {0, 0, 0};
@@ -351,6 +377,7 @@ void orc_interpolate_convol(int tail, int dim, const double *pts, const double *
     }
     for (int m = 0; m < dim; m++) K[m] = 2 * T[m] + 1;
     orc_convolve(tail, dim, vg, wg, sigma, step, size, K, num_iter, alpha, max_dist_weight);
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < total; i++) out[i] = (float)(vg[i] / wg[i] + offset);
     if (vg_out) memcpy(vg_out, vg, sizeof(double) * total);
     if (wg_out) memcpy(wg_out, wg, sizeof(double) * total);
@@ -439,25 +466,22 @@ static void filter_grid(int tail, int dim, double *grid, const int64_t *size, co
                         int64_t num_iter, const double *alpha)
 {
     int64_t W = size[0], H = dim > 1 ? size[1] : 1, D = dim > 2 ? size[2] : 1;
-    int64_t maxlen = W > H ? W : H; if (D > maxlen) maxlen = D;
-    double *a = (double *)malloc(sizeof(double) * maxlen);
-    double *b = (double *)malloc(sizeof(double) * maxlen);
     int64_t nmax = W * H > W * D ? W * H : W * D; if (H * D > nmax) nmax = H * D;
     int64_t *bases = (int64_t *)calloc(nmax > 0 ? nmax : 1, sizeof(int64_t));
     int64_t nl = 0;
     for (int64_t z = 0; z < D; z++) for (int64_t y = 0; y < H; y++) bases[nl++] = (z * H + y) * W;
-    convolve_axis(grid, nl, bases, W, 1, kernel_size[0], num_iter, tail, alpha[0], a, b);
+    convolve_axis(grid, nl, bases, W, 1, kernel_size[0], num_iter, tail, alpha[0]);
     if (dim >= 2) {
         nl = 0;
         for (int64_t z = 0; z < D; z++) for (int64_t x = 0; x < W; x++) bases[nl++] = z * H * W + x;
-        convolve_axis(grid, nl, bases, H, W, kernel_size[1], num_iter, tail, alpha[1], a, b);
+        convolve_axis(grid, nl, bases, H, W, kernel_size[1], num_iter, tail, alpha[1]);
     }
     if (dim >= 3) {
         nl = 0;
         for (int64_t y = 0; y < H; y++) for (int64_t x = 0; x < W; x++) bases[nl++] = y * W + x;
-        convolve_axis(grid, nl, bases, D, W * H, kernel_size[2], num_iter, tail, alpha[2], a, b);
+        convolve_axis(grid, nl, bases, D, W * H, kernel_size[2], num_iter, tail, alpha[2]);
     }
-    free(a); free(b); free(bases);
+    free(bases);
 }
 
 /* out: the filtered copy of grid_in (B x); exposes the self-adjoint operator for tests */

@@ -20,7 +20,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, 'liboracle.so')
+_OMP_PATH = os.path.join(_HERE, 'liboracle_omp.so')
 _lib = None
+_omp = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -35,47 +37,65 @@ def build():
 
 
 def lib():
+    """The checker build (liboracle.so)."""
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        L.orc_half_kernel_size_opt.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _i32p]
-        L.orc_half_kernel_size.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _i32p]
-        L.orc_tail_value.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp]
-        L.orc_sigma_effective.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp]
-        L.orc_threshold.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _i32p, ctypes.c_int64, _dp,
-                                    ctypes.c_double]
-        L.orc_threshold.restype = ctypes.c_double
-        L.orc_normalize_values.argtypes = [_dp, ctypes.c_int64]
-        L.orc_normalize_values.restype = ctypes.c_double
-        L.orc_inject.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int64, _dp, _dp, _i64p]
-        L.orc_accumulate_tail_array.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
-                                                ctypes.c_double]
-        L.orc_accumulate_tail_array.restype = _dp
-        L.orc_accumulate_array.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]
-        L.orc_accumulate_array.restype = _dp
-        L.orc_convolve.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _i64p, _i32p,
-                                   ctypes.c_int64, _dp, ctypes.c_double]
-        L.orc_interpolate_convol.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp, _dp,
-                                             _dp, _i64p, ctypes.c_int64, ctypes.c_double, _fp, _dp, _dp]
-        L.orc_interpolate_naive.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp, _dp, _dp, _i64p, _dp]
-        L.orc_interpolate_radius_bf.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_double, _dp, _dp, _i64p,
-                                                ctypes.c_double, ctypes.c_double, _dp]
-        L.orc_filter_grid.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _i64p, ctypes.c_int64]
-        L.orc_adjoint.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp, _dp, _dp, _i64p,
-                                  ctypes.c_int64, ctypes.c_double, _dp, _dp, _dp]
-        L.orc_lambert_create.argtypes = [ctypes.c_double] * 4 + [_dp]
-        L.orc_lambert_scale.argtypes = [ctypes.c_double, _dp]
-        L.orc_lambert_scale.restype = ctypes.c_double
-        L.orc_lambert_to_map.argtypes = [_dp, _dp, ctypes.c_int64, _dp]
-        L.orc_lambert_to_geo.argtypes = [_dp, _dp, ctypes.c_int64, _dp]
-        L.orc_resample.argtypes = [_fp, _dp, _i64p, _dp, _dp, _i64p, _dp, _fp]
-        L.orc_barnes_s2_opt.argtypes = [_dp, _dp, ctypes.c_int64, _dp, _dp, _dp, _i64p, ctypes.c_int64,
-                                        ctypes.c_double, ctypes.c_int, _fp]
-        L.orc_naive_s2.argtypes = [_dp, _dp, ctypes.c_int64, _dp, _dp, _dp, _i64p, _dp]
-        _lib = L
+        _lib = _bind(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+def omp_lib():
+    """The OpenMP build of the same source (liboracle_omp.so), timed as the CPU baseline;
+    orc_set_threads selects its thread count."""
+    global _omp
+    if _omp is None:
+        if not os.path.exists(_OMP_PATH):
+            build()
+        _omp = _bind(ctypes.CDLL(_OMP_PATH))
+    return _omp
+
+
+def _bind(L):
+    """Argument and result types of the exported functions."""
+    L.orc_half_kernel_size_opt.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _i32p]
+    L.orc_half_kernel_size.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _i32p]
+    L.orc_tail_value.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp]
+    L.orc_sigma_effective.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp]
+    L.orc_threshold.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _i32p, ctypes.c_int64, _dp,
+                                ctypes.c_double]
+    L.orc_threshold.restype = ctypes.c_double
+    L.orc_normalize_values.argtypes = [_dp, ctypes.c_int64]
+    L.orc_normalize_values.restype = ctypes.c_double
+    L.orc_inject.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int64, _dp, _dp, _i64p]
+    L.orc_accumulate_tail_array.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                            ctypes.c_double]
+    L.orc_accumulate_tail_array.restype = _dp
+    L.orc_accumulate_array.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]
+    L.orc_accumulate_array.restype = _dp
+    L.orc_convolve.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _i64p, _i32p,
+                               ctypes.c_int64, _dp, ctypes.c_double]
+    L.orc_interpolate_convol.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp, _dp,
+                                         _dp, _i64p, ctypes.c_int64, ctypes.c_double, _fp, _dp, _dp]
+    L.orc_interpolate_naive.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp, _dp, _dp, _i64p, _dp]
+    L.orc_interpolate_radius_bf.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_double, _dp, _dp, _i64p,
+                                            ctypes.c_double, ctypes.c_double, _dp]
+    L.orc_filter_grid.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _i64p, ctypes.c_int64]
+    L.orc_adjoint.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp, _dp, _dp, _i64p,
+                              ctypes.c_int64, ctypes.c_double, _dp, _dp, _dp]
+    L.orc_lambert_create.argtypes = [ctypes.c_double] * 4 + [_dp]
+    L.orc_lambert_scale.argtypes = [ctypes.c_double, _dp]
+    L.orc_lambert_scale.restype = ctypes.c_double
+    L.orc_lambert_to_map.argtypes = [_dp, _dp, ctypes.c_int64, _dp]
+    L.orc_lambert_to_geo.argtypes = [_dp, _dp, ctypes.c_int64, _dp]
+    L.orc_resample.argtypes = [_fp, _dp, _i64p, _dp, _dp, _i64p, _dp, _fp]
+    L.orc_barnes_s2_opt.argtypes = [_dp, _dp, ctypes.c_int64, _dp, _dp, _dp, _i64p, ctypes.c_int64,
+                                    ctypes.c_double, ctypes.c_int, _fp]
+    L.orc_naive_s2.argtypes = [_dp, _dp, ctypes.c_int64, _dp, _dp, _dp, _i64p, _dp]
+    L.orc_set_threads.argtypes = [ctypes.c_int]
+    L.orc_set_threads.restype = ctypes.c_int
+    return L
 
 
 def _d(a):
@@ -171,17 +191,23 @@ def _common(pts, val, sigma, x0, step, size):
 
 
 def interpolate_convol(pts, val, sigma, x0, step, size, num_iter=4, max_dist=3.5, tail=True,
-                       return_fields=False):
-    """FP64 restatement of _interpolate_opt_convol (tail=True) / _interpolate_convol (tail=False)."""
+                       return_fields=False, threads=None):
+    """FP64 restatement of _interpolate_opt_convol (tail=True) / _interpolate_convol (tail=False).
+    threads: run the OpenMP build (liboracle_omp.so) on that many threads instead of the
+    checker build; the same bits either way."""
+    L = lib()
+    if threads is not None:
+        L = omp_lib()
+        L.orc_set_threads(int(threads))
     pts, dim, val, sg, x0v, st, sz = _common(pts, val, sigma, x0, step, size)
     mdw = math.exp(-max_dist ** 2 / 2)
     shape = tuple(int(s) for s in sz[::-1])
     out = np.empty(shape, dtype=np.float32)
     vg = np.empty(shape, dtype=np.float64) if return_fields else None
     wg = np.empty(shape, dtype=np.float64) if return_fields else None
-    lib().orc_interpolate_convol(int(tail), dim, _d(pts), _d(val), len(val), _d(sg), _d(x0v), _d(st),
-                                 sz.ctypes.data_as(_i64p), num_iter, mdw, out.ctypes.data_as(_fp),
-                                 _d(vg) if return_fields else None, _d(wg) if return_fields else None)
+    L.orc_interpolate_convol(int(tail), dim, _d(pts), _d(val), len(val), _d(sg), _d(x0v), _d(st),
+                             sz.ctypes.data_as(_i64p), num_iter, mdw, out.ctypes.data_as(_fp),
+                             _d(vg) if return_fields else None, _d(wg) if return_fields else None)
     if return_fields:
         return out, vg, wg
     return out

@@ -507,3 +507,26 @@ def test_basic_ranges_1d_2d(orc, known_answers):
                 r = orc.interpolate_convol(data[:, :2], data[:, 2], sigma, 0.0, d2['step'], tuple(d2['size']),
                                            num_iter=n, tail=tail)
                 assert np.all(((r >= -d2['eps']) & (r <= 1 + d2['eps'])) | np.isnan(r))
+
+
+# ---------------------------------------------------------------------------
+# the OpenMP build timed as the CPU baseline
+
+@pytest.mark.parametrize('threads', [1, 3, 8])
+def test_omp_build_matches_checker(orc, threads):
+    """liboracle_omp.so (-O3, OpenMP over the lines of each pass) gives the checker's bits, in 2D
+    (config 1, both methods) and 3D, on any thread count."""
+    from fastbarnes import workloads
+    c = workloads.config1(seed=5)
+    c3 = workloads.config4(n=3000, size=(40, 36, 28))   # plain kernel 21 < 28
+    for cc in (c, c3):
+        for tail in (True, False):
+            a, va, wa = orc.interpolate_convol(cc['pts'], cc['val'], cc['sigma'], cc['x0'], cc['step'], cc['size'],
+                                               num_iter=cc['num_iter'], max_dist=cc['max_dist'], tail=tail,
+                                               return_fields=True)
+            b, vb, wb = orc.interpolate_convol(cc['pts'], cc['val'], cc['sigma'], cc['x0'], cc['step'], cc['size'],
+                                               num_iter=cc['num_iter'], max_dist=cc['max_dist'], tail=tail,
+                                               return_fields=True, threads=threads)
+            assert np.array_equal(va.view(np.uint64), vb.view(np.uint64))
+            assert np.array_equal(np.isnan(wa), np.isnan(wb))
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
