@@ -416,6 +416,7 @@ struct fb_plan {
     int64_t lslots[3] = {0, 0, 0};
     double *d_offsets = nullptr, *d_off_part = nullptr;
     int64_t max_n = 0;
+    int64_t max_rows = 0;   // most x-lines of any field
     uint32_t *d_keys = nullptr, *d_vals = nullptr, *d_keys2 = nullptr, *d_vals2 = nullptr;
     uint32_t *d_line_count = nullptr, *d_line_start = nullptr, *d_scan_tmp = nullptr;
     uint32_t *d_skeys = nullptr, *d_big_list = nullptr, *d_big_count = nullptr;   // row buckets
@@ -724,7 +725,7 @@ int run_pass(fb_plan *P, int axis, bool inject, const uint32_t *vals, const doub
 // arena: the plan's device buffers are carved from [arena, arena + arena_cap) instead of the
 // pool (fb_barnes_noalloc)
 int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precision, void *arena = nullptr,
-                size_t arena_cap = 0)
+                size_t arena_cap = 0, bool grids = true)
 {
     *out = nullptr;
     if (!fields || nfields < 1) return fail(FB_ERR_INVALID_ARGUMENT, "no fields");
@@ -793,7 +794,10 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
     TRY(P->upload(&P->d_cell_prefix, cell_prefix));
     for (int a = 0; a < 3; ++a) TRY(P->upload(&P->d_lslot_prefix[a], lslot[a]));
     TRY(P->alloc(&P->d_offsets, nfields));
-    for (const FieldDev &d : P->fh) P->max_n = std::max(P->max_n, d.n);
+    for (const FieldDev &d : P->fh) {
+        P->max_n = std::max(P->max_n, d.n);
+        P->max_rows = std::max(P->max_rows, d.size[1] * d.size[2]);
+    }
     TRY(P->alloc(&P->d_off_part, offsets_part_elems(nfields, P->max_n)));
     TRY(P->alloc(&P->d_keys, P->nentries));
     TRY(P->alloc(&P->d_vals, P->nentries));
@@ -807,7 +811,9 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
     TRY(P->alloc(&P->d_line_count, P->nxlines));
     TRY(P->alloc(&P->d_line_start, P->nxlines));
     TRY(P->alloc(&P->d_scan_tmp, scan_temp_elems(P->nxlines)));
-    if (!fast) {
+    if (!grids) {
+        // the sort and injection stages only (fb_inject): the caller's grids take the sums
+    } else if (!fast) {
         TRY(P->alloc(&P->d_vg, P->ncells));
         TRY(P->alloc(&P->d_wg, P->ncells));
         TRY(P->alloc(&P->d_vs, P->ncells));
@@ -832,13 +838,11 @@ void stage_mark(fb_plan *P, hipStream_t s, const char *name)
     P->stage_names.push_back(name);
 }
 
-int plan_run(fb_plan *P, const double *pts, const double *val, float *out, float *wsave, hipStream_t s)
+// The sort stage of a run: the value offsets and the points ordered by base row, (base cell,
+// point index); *sorted receives the sorted point indices.
+int plan_sort(fb_plan *P, const double *pts, const double *val, hipStream_t s, const uint32_t **sorted)
 {
     int rc;
-    for (hipEvent_t e : P->events) (void)hipEventDestroy(e);
-    P->events.clear();
-    P->stage_names.clear();
-    stage_mark(P, s, "begin");
     // Per-row point counts by integer atomics and their exclusive scan give the rows' ranges of
     // the sorted list.  Fields of at most kLocalSortMax points: one workgroup sorts each field
     // in LDS.  Larger fields: every point goes to its atomic slot of its row's range, then each
@@ -855,7 +859,7 @@ int plan_run(fb_plan *P, const double *pts, const double *val, float *out, float
     } else {
         if ((rc = launch_offsets(P->d_fields, P->nfields, P->max_n, val, P->d_off_part, P->d_offsets, s))) return rc;
         if ((rc = launch_point_keys(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints, pts, P->d_keys, P->d_vals,
-                                    P->d_line_count, local ? nullptr : P->d_vals, local ? nullptr : P->d_keys2, s)))
+                                    P->d_line_count, local ? nullptr : P->d_vals, local ? nullptr : P->d_keys2, P->max_rows, s)))
             return rc;
     }
     uint32_t *vs = P->d_vals2;
@@ -874,6 +878,19 @@ int plan_run(fb_plan *P, const double *pts, const double *val, float *out, float
                                   P->d_vals, s)))
             return rc;
     }
+    *sorted = vs;
+    return FB_OK;
+}
+
+int plan_run(fb_plan *P, const double *pts, const double *val, float *out, float *wsave, hipStream_t s)
+{
+    int rc;
+    for (hipEvent_t e : P->events) (void)hipEventDestroy(e);
+    P->events.clear();
+    P->stage_names.clear();
+    stage_mark(P, s, "begin");
+    const uint32_t *vs = nullptr;
+    if ((rc = plan_sort(P, pts, val, s, &vs))) return rc;
     P->sorted_vals = vs;
     stage_mark(P, s, "sort");
     if (P->precision == FB_PREC_FP64) {
@@ -1481,6 +1498,83 @@ int fb_barnes_noalloc(int method, int precision, int dim, int64_t n, const doubl
     const hipStream_t s = (hipStream_t)stream;
     rc = plan_run(P, d_pts, d_val, d_out, nullptr, s);
     // wait in any case: kernels queued before an error may still use the scratch array
+    const hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        if (!rc) rc = fail(FB_ERR_HIP, std::string("execution failed: ") + hipGetErrorString(e));
+    }
+    delete P;
+    return rc;
+}
+
+int fb_inject(int dim, int64_t n, const double *pts, const double *val, const double *x0, const double *step,
+              const int64_t *size, double *vg, double *wg, unsigned flags, void *stream)
+{
+    if (dim < 1 || dim > 3)
+        return fail(FB_ERR_INVALID_ARGUMENT, "Barnes interpolation supports only sample points in dimensions 1, 2 or 3");
+    if (n < 0 || (n > 0 && (!pts || !val)) || !x0 || !step || !size || !vg || !wg)
+        return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument");
+    int64_t cells = 1;
+    bool empty = n == 0;
+    for (int m = 0; m < dim; ++m) {
+        if (size[m] < 1) return fail(FB_ERR_INVALID_ARGUMENT, "grid size must be positive");
+        if (!finite_pos(step[m]) || !std::isfinite(x0[m]))
+            return fail(FB_ERR_INVALID_ARGUMENT, "step must be finite and positive, x0 finite");
+        cells *= size[m];
+        empty = empty || size[m] < 2;   // no point lies below size-1 along that axis
+    }
+    if (empty) return FB_OK;
+    const bool on_dev = flags & FB_FLAG_DEVICE_PTRS;
+    int device = 0;
+    FB_HIP_CHECK(hipGetDevice(&device));
+    hipStream_t s = stream ? (hipStream_t)stream : lib_stream(device);
+    Scratch sc;
+    sc.device = device;
+    sc.on(s);
+    int rc;
+    const double *d_pts = pts, *d_val = val;
+    double *d_vg = vg, *d_wg = wg;
+    if (on_dev) {
+        if ((rc = check_device_ptr(pts, "pts")) || (rc = check_device_ptr(val, "val")) ||
+            (rc = check_device_ptr(vg, "vg")) || (rc = check_device_ptr(wg, "wg")))
+            return rc;
+    } else {
+        double *a, *b;
+        if ((rc = sc.get(&a, n * dim)) || (rc = sc.get(&b, n)) || (rc = sc.get(&d_vg, cells)) ||
+            (rc = sc.get(&d_wg, cells)))
+            return rc;
+        FB_HIP_CHECK(hipMemcpyAsync(a, pts, n * dim * sizeof(double), hipMemcpyHostToDevice, s));
+        FB_HIP_CHECK(hipMemcpyAsync(b, val, n * sizeof(double), hipMemcpyHostToDevice, s));
+        FB_HIP_CHECK(hipMemcpyAsync(d_vg, vg, cells * sizeof(double), hipMemcpyHostToDevice, s));
+        FB_HIP_CHECK(hipMemcpyAsync(d_wg, wg, cells * sizeof(double), hipMemcpyHostToDevice, s));
+        d_pts = a;
+        d_val = b;
+    }
+    // A one-field FP64 plan for its sort stage; its kernel parameters are unused (sigma =
+    // step/10 and one iteration keep them valid for any grid of at least 2 nodes per axis).
+    double sig[3];
+    for (int m = 0; m < dim; ++m) sig[m] = 0.1 * step[m];
+    const fb_field f = single_field(FB_METHOD_OPTIMIZED_CONVOLUTION, dim, n, sig, x0, step, size, 1, 0.0);
+    fb_plan *P = nullptr;
+    if ((rc = plan_create(&P, &f, 1, FB_PREC_FP64, nullptr, 0, false))) return rc;
+    const uint32_t *vs = nullptr;
+    rc = plan_sort(P, d_pts, d_val, s, &vs);
+    // the values are injected as given: offset 0 (one field's grid starts at cell 0, x-fastest,
+    // exactly the caller's layout)
+    if (!rc && hipMemsetAsync(P->d_offsets, 0, sizeof(double), s) != hipSuccess) {
+        (void)hipGetLastError();
+        rc = fail(FB_ERR_HIP, "clearing the offset failed");
+    }
+    if (!rc)
+        rc = launch_exact_inject(P->d_fields, P->d_xline_prefix, 1, P->nxlines, vs, P->d_line_start,
+                                 P->d_line_count, d_pts, d_val, P->d_offsets, d_vg, d_wg, s);
+    if (!rc && !on_dev) {
+        if (hipMemcpyAsync(vg, d_vg, cells * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipMemcpyAsync(wg, d_wg, cells * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) {
+            (void)hipGetLastError();
+            rc = fail(FB_ERR_HIP, "copy of the grids failed");
+        }
+    }
     const hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
         (void)hipGetLastError();

@@ -237,6 +237,91 @@ __global__ __launch_bounds__(kBlock) void k_point_keys_offsets(const FieldDev *_
         offsets_block(fields, val, offsets, (int)(blockIdx.x - nkb));
 }
 
+// k_point_keys for large point sets: each workgroup takes ppt*kBlock consecutive points and,
+// when the rows of the fields they belong to number at most kKeysBins, counts them per row in
+// LDS first; one global atomic per row the workgroup touches then reserves the row's slots for
+// all of them (instead of one contended atomic per point), and each point's slot is that base
+// plus its rank among the workgroup's points of the row.  The slots come in another order than
+// with one atomic per point; the row sort orders every row by (base cell, point index), so the
+// sorted list is the same.  Workgroups whose rows do not fit count with one atomic per point.
+constexpr int kKeysMaxPpt = 16;     // points per thread, at most
+constexpr int kKeysBins = 8192;     // LDS row counters of a workgroup (32 KB)
+
+__global__ __launch_bounds__(kBlock) void k_point_keys_hist(const FieldDev *__restrict__ fields,
+                                                            const int64_t *__restrict__ pt_prefix, int nfields,
+                                                            int64_t npoints, int ppt, const double *__restrict__ pts,
+                                                            uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                            uint32_t *__restrict__ row_count,
+                                                            uint32_t *__restrict__ slot, uint32_t *__restrict__ rowg)
+{
+    __shared__ uint32_t hist[kKeysBins];
+    __shared__ int64_t s_rbase;
+    __shared__ int s_span;
+    const int64_t p0 = (int64_t)blockIdx.x * ppt * kBlock;
+    const int64_t p1 = min(p0 + (int64_t)ppt * kBlock, npoints);
+    if (threadIdx.x == 0) {
+        const FieldDev &A = fields[find_seg(pt_prefix, nfields, p0)];
+        const FieldDev &B = fields[find_seg(pt_prefix, nfields, p1 - 1)];
+        const int64_t r0 = A.xline_off;
+        const int64_t span = B.xline_off + B.size[1] * B.size[2] - r0;
+        s_rbase = r0;
+        s_span = span <= kKeysBins ? (int)span : -1;
+    }
+    __syncthreads();
+    const int64_t rbase = s_rbase;
+    const int span = s_span;
+    for (int b = threadIdx.x; b < span; b += kBlock) hist[b] = 0u;
+    __syncthreads();
+    int rr[kKeysMaxPpt];
+    uint32_t lr[kKeysMaxPpt];
+#pragma unroll
+    for (int k = 0; k < kKeysMaxPpt; ++k) {
+        rr[k] = -1;
+        lr[k] = 0u;
+        const int64_t i = p0 + (int64_t)k * kBlock + threadIdx.x;
+        if (k >= ppt || i >= p1) continue;
+        const int f = find_seg(pt_prefix, nfields, i);
+        const FieldDev &F = fields[f];
+        PointPos pp;
+        const bool inside = point_pos(F, pts + i * F.dim, pp);
+        if (!slot) vals[i] = (uint32_t)i;
+        if (!inside) {
+            keys[i] = kInvalidKey;
+            if (rowg) rowg[i] = kInvalidKey;
+            continue;
+        }
+        const int64_t row = pp.i[2] * F.size[1] + pp.i[1];
+        keys[i] = (uint32_t)(F.cell_off + row * F.size[0] + pp.i[0]);
+        const int64_t grow = F.xline_off + row;
+        if (span >= 0) {
+            rr[k] = (int)(grow - rbase);
+            lr[k] = atomicAdd(&hist[rr[k]], 1u);
+        } else {
+            const uint32_t sl = atomicAdd(&row_count[grow], 1u);
+            if (slot) {
+                slot[i] = sl;
+                rowg[i] = (uint32_t)grow;
+            }
+        }
+    }
+    if (span < 0) return;   // uniform for the workgroup
+    __syncthreads();
+    // one global atomic per touched row; the LDS counter becomes the row's first slot
+    for (int b = threadIdx.x; b < span; b += kBlock) {
+        const uint32_t c = hist[b];
+        if (c) hist[b] = atomicAdd(&row_count[rbase + b], c);
+    }
+    if (!slot) return;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kKeysMaxPpt; ++k) {
+        if (rr[k] < 0) continue;
+        const int64_t i = p0 + (int64_t)k * kBlock + threadIdx.x;
+        slot[i] = hist[rr[k]] + lr[k];
+        rowg[i] = (uint32_t)(rbase + rr[k]);
+    }
+}
+
 // The sub-streams of x-line (y, z): for q < 2*nrows, the points of base row (y - dy, z - dz)
 // contribute their corner (bx, dy, dz) at x = base x + bx, with bx = q & 1, dy = (q >> 1) & 1,
 // dz = q >> 2.  Within a base row the sorted points ascend in base x (point order for equal
@@ -1590,11 +1675,23 @@ int launch_point_keys_offsets(const FieldDev *d_fields, const int64_t *d_pt_pref
 
 int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
                       const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_row_count,
-                      uint32_t *d_slot, uint32_t *d_rowg, hipStream_t s)
+                      uint32_t *d_slot, uint32_t *d_rowg, int64_t max_rows, hipStream_t s)
 {
     if (npoints <= 0) return FB_OK;
-    hipLaunchKernelGGL(k_point_keys, dim3(nblocks(npoints)), dim3(kBlock), 0, s, d_fields, d_pt_prefix, nfields,
-                       npoints, d_pts, d_keys, d_vals, d_row_count, d_slot, d_rowg);
+    // points per thread: enough for each workgroup to see many points per row, while keeping
+    // at least ~512 workgroups; small point sets, and fields of more rows than half the LDS
+    // counters (whose rows then rarely collide), take one atomic per point
+    const int64_t q = npoints / (512 * (int64_t)kBlock);
+    const int64_t ppt = q < kKeysMaxPpt ? q : kKeysMaxPpt;
+    if (ppt >= 4 && d_row_count && max_rows <= kKeysBins / 2) {
+        const int64_t per = ppt * kBlock;
+        hipLaunchKernelGGL(k_point_keys_hist, dim3((unsigned)((npoints + per - 1) / per)), dim3(kBlock), 0, s,
+                           d_fields, d_pt_prefix, nfields, npoints, (int)ppt, d_pts, d_keys, d_vals, d_row_count,
+                           d_slot, d_rowg);
+    } else {
+        hipLaunchKernelGGL(k_point_keys, dim3(nblocks(npoints)), dim3(kBlock), 0, s, d_fields, d_pt_prefix, nfields,
+                           npoints, d_pts, d_keys, d_vals, d_row_count, d_slot, d_rowg);
+    }
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }

@@ -26,10 +26,11 @@ int launch_offsets(const FieldDev *d_fields, int nfields, int64_t max_n, const d
                    double *d_offsets, hipStream_t stream);
 // one (base cell, point) pair per point and, unless d_row_count is NULL, per-base-row point
 // counts (row = x-line index) by integer atomics; with d_slot/d_rowg (row buckets) each point's
-// slot in its row and its row instead of the point indices
+// slot in its row and its row instead of the point indices.  max_rows: the most x-lines of any
+// field (fields of at most 4096 count their points per row in LDS first)
 int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
                       const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_row_count,
-                      uint32_t *d_slot, uint32_t *d_rowg, hipStream_t stream);
+                      uint32_t *d_slot, uint32_t *d_rowg, int64_t max_rows, hipStream_t stream);
 // launch_point_keys and, when the offsets take one workgroup per field (offsets_single_pass),
 // launch_offsets in one launch
 bool offsets_single_pass(int64_t max_n, const double *d_part);

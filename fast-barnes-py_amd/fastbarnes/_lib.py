@@ -70,6 +70,8 @@ SIGNATURES = {
                                          _dp, _i64p, ctypes.c_int, ctypes.c_double, _vp, _vp]),
     'fb_barnes_s2': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, _vp, _vp, _dp, _dp, _dp, _i64p,
                                     ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, ctypes.c_uint, _vp]),
+    'fb_inject': (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _vp, _vp, _dp, _dp, _i64p, _vp, _vp, ctypes.c_uint,
+                                 _vp]),
     'fb_accumulate_lines': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
                                            ctypes.c_int, ctypes.c_double, _vp, _vp, ctypes.c_uint, _vp]),
     'fb_convolve_grid': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i64p, _i32p, ctypes.c_int, _dp,

@@ -311,3 +311,71 @@ def _convolve_2d(vg, wg, sigma, step, size, kernel_size, num_iter, max_dist_weig
 def _convolve_3d(vg, wg, sigma, step, size, kernel_size, num_iter, max_dist_weight):
     """In-place plain convolution + NaN mask of 3D fields (interpolation.py:678-724)."""
     _convolve_grid(False, vg, wg, sigma, step, size, kernel_size, num_iter, None, max_dist_weight)
+
+
+def _normalize_values(val):
+    """Offsets the values, in place, so that their range is centred on 0; returns the offset
+    (interpolation.py:204-212; np.amin/np.amax propagate NaN)."""
+    offset = (np.amin(val) + np.amax(val)) / 2.0
+    val -= offset
+    return offset
+
+
+def _inject_data(dim, vg, wg, pts, val, x0, step, size):
+    for a in (vg, wg):
+        if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not a.flags.c_contiguous:
+            raise RuntimeError('grid arrays must be C-contiguous float64')
+        if a.size != int(np.prod(np.asarray(size, dtype=np.int64))):
+            raise RuntimeError('grid arrays do not match size')
+    p = np.ascontiguousarray(np.asarray(pts, dtype=np.float64).reshape(len(pts), -1)[:, :dim])
+    v = np.ascontiguousarray(val, dtype=np.float64)
+    x0v = np.ascontiguousarray(np.asarray(x0, dtype=np.float64).reshape(-1)[:dim])
+    st = np.ascontiguousarray(np.asarray(step, dtype=np.float64).reshape(-1)[:dim])
+    sz = np.ascontiguousarray(np.asarray(size, dtype=np.int64).reshape(-1)[:dim])
+    check(lib().fb_inject(dim, len(v), p.ctypes.data, v.ctypes.data, _dptr(x0v), _dptr(st),
+                          sz.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), vg.ctypes.data, wg.ctypes.data, 0,
+                          None))
+
+
+def _inject_data_1d(vg, wg, pts, val, x0, step, size):
+    """Adds the linear corner weights of the samples to vg (times val) and wg, in place
+    (interpolation.py:218-238), on the GPU; the sums equal the reference's bit for bit."""
+    _inject_data(1, vg, wg, pts, val, x0, step, size)
+
+
+def _inject_data_2d(vg, wg, pts, val, x0, step, size):
+    """Adds the bilinear corner weights of the samples to vg (times val) and wg, in place
+    (interpolation.py:241-272)."""
+    _inject_data(2, vg, wg, pts, val, x0, step, size)
+
+
+def _inject_data_3d(vg, wg, pts, val, x0, step, size):
+    """Adds the trilinear corner weights of the samples to vg (times val) and wg, in place
+    (interpolation.py:275-322)."""
+    _inject_data(3, vg, wg, pts, val, x0, step, size)
+
+
+def _interpolate_conv_private(mcode, pts, val, sigma, x0, step, size, num_iter, max_dist_weight):
+    dim = len(size)
+    orig = np.array(val, dtype=np.float64, copy=True)
+    out = _run(mcode, dim, np.asarray(pts, dtype=np.float64).reshape(len(orig), -1), orig, _vec(sigma, dim),
+               _vec(x0, dim), _vec(step, dim), tuple(size), num_iter, max_dist_weight, 0.0,
+               _precision_code(_precision_private))
+    # like the reference, the caller's val is left normalised (the GPU formed the same offset
+    # from the unmodified copy)
+    _normalize_values(val)
+    return out
+
+
+def _interpolate_opt_convol(pts, val, sigma, x0, step, size, num_iter, max_dist_weight):
+    """Algorithm B with the tailed rectangular kernel (interpolation.py:328-367): the float32
+    field of shape size[::-1].  As in the reference, val is normalised in place."""
+    return _interpolate_conv_private(_lib.FB_METHOD_OPTIMIZED_CONVOLUTION, pts, val, sigma, x0, step, size,
+                                     num_iter, max_dist_weight)
+
+
+def _interpolate_convol(pts, val, sigma, x0, step, size, num_iter, max_dist_weight):
+    """Algorithm B with the plain rectangular kernel (interpolation.py:575-612); val is
+    normalised in place."""
+    return _interpolate_conv_private(_lib.FB_METHOD_CONVOLUTION, pts, val, sigma, x0, step, size, num_iter,
+                                     max_dist_weight)

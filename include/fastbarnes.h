@@ -122,6 +122,16 @@ int fb_barnes_s2(int method, int precision, int64_t n, const double *pts, const 
 
 /* ---- line and grid primitives behind the reference's private helpers ---- */
 
+/* _inject_data_1d/2d/3d  interpolation.py:218-322: adds the 2^dim corner weights w of every
+ * sample point (bilinear / trilinear, algorithm B.1) as w*val[k] to vg and w to wg, FP64 grids
+ * of shape size[::-1] (x fastest) whose existing contents are kept.  Each cell receives its
+ * contributions in increasing point index, the reference's order, so the sums are bitwise the
+ * reference's.  val is used as given (the caller has normalised it).  Points outside
+ * [x0, x0 + (size-1)*step) along any axis are skipped.  flags, stream as for fb_barnes; the
+ * call returns after vg and wg are written. */
+int fb_inject(int dim, int64_t n, const double *pts, const double *val, const double *x0, const double *step,
+              const int64_t *size, double *vg, double *wg, unsigned flags, void *stream);
+
 /* num_iter-fold box filter of `nlines` lines of length len (row-major nlines x len, FP64):
  *   alpha-tailed  _accumulate_tail_array  interpolation.py:484-533  (tail != 0)
  *   plain         _accumulate_array       interpolation.py:729-772  (tail == 0)

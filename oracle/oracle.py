@@ -231,6 +231,18 @@ def interpolate_radius(pts, val, sigma, x0, step, size, max_dist=3.5, min_weight
     return out
 
 
+def inject_raw(vg, wg, pts, val, x0, step, size):
+    """_inject_data_*d as the reference calls it: adds into the existing vg, wg (float64,
+    shape size[::-1]) with val used as given."""
+    pts = _pts(pts)
+    dim = pts.shape[1]
+    v = np.ascontiguousarray(val, dtype=np.float64)
+    sz = np.asarray(size, dtype=np.int64).reshape(-1)
+    assert vg.dtype == np.float64 and wg.dtype == np.float64 and vg.flags.c_contiguous and wg.flags.c_contiguous
+    lib().orc_inject(dim, _d(vg), _d(wg), _d(pts), _d(v), len(v), _d(_vec(x0, dim)), _d(_vec(step, dim)),
+                     sz.ctypes.data_as(_i64p))
+
+
 def inject(pts, val, x0, step, size):
     """Returns (vg, wg) after _normalize_values + _inject_data_*d, and the offset."""
     pts = _pts(pts)
