@@ -428,6 +428,7 @@ struct fb_plan {
     std::map<std::pair<int, int64_t>, PassChoice> seg_choice;   // (axis, line length) -> autotuned choice
     const uint32_t *sorted_vals = nullptr;      // sorted point indices of the last run
     int allow_blocked = 1;                      // fb_plan_set_blocked
+    bool inplace_passes = false;                // every pass filters its planes in place (no chunks)
 
     int timing = 0;
     std::vector<hipEvent_t> events;
@@ -526,6 +527,48 @@ struct fb_plan {
 
 namespace fb {
 namespace {
+// Chunks of an FP32C line pass whose lines are longer than the pass takes (no segment length
+// and workgroup size fits them): lines of length L run as chunks that each store C nodes and
+// load H = num_iter*(T+1) more on either side within the line; C is balanced over the fewest
+// chunks of at most Lc = C + 2H nodes that the pass takes.  Lc = 0: the whole line fits.
+struct ChunkGeom {
+    int64_t Lc = 0, C = 0, H = 0;
+};
+
+int chunk_geom(int64_t L, int T, int num_iter, ChunkGeom &g)
+{
+    g = ChunkGeom{};
+    bool ok = false;
+    choice_cost(L, T, model_choice(L, T), &ok);
+    if (ok) return FB_OK;
+    const int64_t H = (int64_t)num_iter * (T + 1);
+    for (int64_t Lc : {8192, 6144, 4096, 3072, 2048}) {
+        if (Lc >= L || Lc <= 2 * H + 256) continue;
+        choice_cost(Lc, T, model_choice(Lc, T), &ok);
+        if (!ok) continue;
+        const int64_t nc = (L + (Lc - 2 * H) - 1) / (Lc - 2 * H);
+        g.C = (L + nc - 1) / nc;
+        g.H = H;
+        g.Lc = g.C + 2 * H;
+        return FB_OK;
+    }
+    return fail(FB_ERR_UNSUPPORTED, "lines of length " + std::to_string(L) + " with half kernel " + std::to_string(T) +
+                                        " and " + std::to_string(num_iter) +
+                                        " iterations fit no FP32C line pass, whole or in chunks; use precision fp64");
+}
+
+// Length the fast pass along `axis` lays its LDS out for: the field's line length, or its
+// chunks' when the lines run in chunks (0 and an error when they cannot).
+int64_t pass_len(const FieldDev &F, int axis)
+{
+    ChunkGeom g;
+    const std::string keep = last_error();
+    const int rc = chunk_geom(F.size[axis], F.T[axis], F.num_iter, g);
+    set_error(keep);
+    if (rc) return 0;
+    return g.Lc ? g.Lc : F.size[axis];
+}
+
 // the pass choice of an FP32C plan for lines of length L along `axis` (autotuned, forced or the
 // cost model's)
 PassChoice pass_choice(const fb_plan *P, int axis, int64_t L, int T)
@@ -542,6 +585,7 @@ bool field_blocked(const fb_plan *P, int f)
 {
     const FieldDev &F = P->fh[f];
     if (!P->allow_blocked || F.dim != 2 || F.size[0] % 4 || F.cell_off % 4) return false;
+    if (pass_len(F, 0) != F.size[0] || pass_len(F, 1) != F.size[1]) return false;   // chunked lines
     const PassChoice cx = pass_choice(P, 0, F.size[0], F.T[0]);
     const PassChoice cy = pass_choice(P, 1, F.size[1], F.T[1]);
     if (cx.kind != kPassSlide || cy.kind != kPassSlide) return false;
@@ -634,18 +678,34 @@ int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<Pa
     for (int f = 0; f < P->nfields; ++f) {
         const FieldDev &F = P->fh[f];
         const int64_t W = F.size[0], H = F.size[1], D = F.size[2];
-        const int64_t L = F.size[axis];
-        const PassChoice ch = pass_choice(P, axis, L, F.T[axis]);
+        ChunkGeom cg;
+        int rc = chunk_geom(F.size[axis], F.T[axis], F.num_iter, cg);
+        if (rc) return rc;
+        // a chunk's halo is read from the pass's input while other chunks store into it: only
+        // passes whose input and output differ (the injecting first pass, the last pass) chunk
+        if (cg.Lc && P->inplace_passes)
+            return fail(FB_ERR_UNSUPPORTED, "lines of length " + std::to_string(F.size[axis]) +
+                                                " exceed the FP32C line pass here; use precision fp64");
+        if (cg.Lc && axis != 0 && axis != P->dim - 1)
+            return fail(FB_ERR_UNSUPPORTED, "FP32C takes y lines longer than " + std::to_string(cg.Lc - 2 * cg.H) +
+                                                " in 2D only; use precision fp64");
+        const int64_t L = cg.Lc ? cg.Lc : F.size[axis];
+        PassChoice ch = pass_choice(P, axis, L, F.T[axis]);
+        if (cg.Lc && ch.kind != kPassSlide) ch = model_choice(L, F.T[axis]);   // chunks: the sliding pass
         const int seg = ch.seg;
         int lpl = 1, lines = 1, rpad = 0, wgs = 0;
         size_t l = 0;
-        int rc = ch.kind == kPassPrefix ? prefix_geometry(L, F.T[axis], seg, ch.lpf, lpl, lines, l, wgs)
-                                        : pass_geometry(L, F.T[axis], seg, ch.lpf, lpl, lines, rpad, l);
+        rc = ch.kind == kPassPrefix ? prefix_geometry(L, F.T[axis], seg, ch.lpf, lpl, lines, l, wgs)
+                                    : pass_geometry(L, F.T[axis], seg, ch.lpf, lpl, lines, rpad, l);
         if (rc) return rc;
+        const int nch = cg.Lc ? (int)((F.size[axis] + cg.C - 1) / cg.C) : 1;
         const int gkey = (ch.kind * 1024 + seg) * 1024 + ch.lpf;
         lds_of[gkey] = std::max(lds_of[gkey], l);
         PassField &p = pf[f];
         p.len = L;
+        p.full_len = F.size[axis];
+        p.chunk = (int32_t)cg.C;
+        p.halo = (int32_t)cg.H;
         p.T = F.T[axis];
         p.lpl = lpl;
         p.lines = lines;
@@ -663,18 +723,21 @@ int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<Pa
             p.elem_stride = 1;
             const int64_t nlines = H * D;
             for (int64_t l0 = 0; l0 < nlines; l0 += lines)
-                items.push_back({f, (int32_t)std::min<int64_t>(lines, nlines - l0), l0 * W});
+                for (int c = 0; c < nch; ++c)
+                    items.push_back({f, (int32_t)std::min<int64_t>(lines, nlines - l0), l0 * W, c});
         } else if (axis == 1) {
             p.line_stride = 1;
             p.elem_stride = W;
             for (int64_t z = 0; z < D; ++z)
                 for (int64_t x0 = 0; x0 < W; x0 += lines)
-                    items.push_back({f, (int32_t)std::min<int64_t>(lines, W - x0), z * W * H + x0});
+                    for (int c = 0; c < nch; ++c)
+                        items.push_back({f, (int32_t)std::min<int64_t>(lines, W - x0), z * W * H + x0, c});
         } else {
             p.line_stride = 1;
             p.elem_stride = W * H;
             for (int64_t c0 = 0; c0 < W * H; c0 += lines)
-                items.push_back({f, (int32_t)std::min<int64_t>(lines, W * H - c0), c0});
+                for (int c = 0; c < nch; ++c)
+                    items.push_back({f, (int32_t)std::min<int64_t>(lines, W * H - c0), c0, c});
         }
     }
     work.clear();
@@ -1128,6 +1191,12 @@ int fb_plan_backward(fb_plan *P, const double *d_pts, const double *d_val, const
         return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument");
     if (P->precision != FB_PREC_FP32C)
         return fail(FB_ERR_UNSUPPORTED, "the training forward and the adjoint need a plan of precision FP32C");
+    // the adjoint's later passes filter their planes in place, which chunked lines cannot
+    for (const FieldDev &F : P->fh)
+        for (int a = 1; a < P->dim; ++a)
+            if (pass_len(F, a) != F.size[a])
+                return fail(FB_ERR_UNSUPPORTED, "the adjoint takes lines of at most the FP32C line pass's length "
+                                                "along y and z");
     hipStream_t s = (hipStream_t)stream;
     int rc = launch_offsets(P->d_fields, P->nfields, P->max_n, d_val, P->d_off_part, P->d_offsets, s);
     // the same scaled line passes; the first one forms a = g/W and b = -a(f - offset) in its tile
@@ -1201,9 +1270,12 @@ int fb_plan_autotune(fb_plan *P, const double *d_pts, const double *d_val, float
     if (rc || rc2) return rc ? rc : rc2;
     FB_HIP_CHECK(hipStreamSynchronize(s));
     for (int a = 0; a < P->dim; ++a) {
-        std::map<int64_t, std::vector<char>> classes;   // line length -> fields with it
+        std::map<int64_t, std::vector<char>> classes;   // pass line length -> fields with it
+        std::map<int64_t, bool> chunked;                 // ... some of them in chunks
         for (int f = 0; f < P->nfields; ++f) {
-            std::vector<char> &sel = classes[P->fh[f].size[a]];
+            const int64_t len = pass_len(P->fh[f], a);
+            if (len != P->fh[f].size[a]) chunked[len] = true;
+            std::vector<char> &sel = classes[len];
             sel.resize(P->nfields, 0);
             sel[f] = 1;
         }
@@ -1219,8 +1291,9 @@ int fb_plan_autotune(fb_plan *P, const double *d_pts, const double *d_val, float
                 return x.seg == y.seg && x.lpf == y.lpf && x.kind == y.kind;
             };
             for (int lpf : kLpfs) {
-                for (int seg : kPfxSegs)
-                    if (!same({seg, lpf, kPassPrefix}, base)) cands.push_back({seg, lpf, kPassPrefix});
+                for (int seg : kPfxSegs)   // chunked lines run the sliding pass only
+                    if (!chunked[cl.first] && !same({seg, lpf, kPassPrefix}, base))
+                        cands.push_back({seg, lpf, kPassPrefix});
                 for (int seg : kSegs)
                     if (!same({seg, lpf, kPassSlide}, base)) cands.push_back({seg, lpf, kPassSlide});
             }
@@ -1277,7 +1350,9 @@ int fb_plan_choose_pass(fb_plan *P, int axis, int64_t len, int kind, int seg, in
     const PassChoice ch{seg, threads / 2, kind};
     bool any = false;
     for (int f = 0; f < P->nfields; ++f) {
-        if (P->fh[f].size[axis] != len) continue;
+        if (pass_len(P->fh[f], axis) != len) continue;
+        if (kind != kPassSlide && len != P->fh[f].size[axis])
+            return fail(FB_ERR_UNSUPPORTED, "lines that run in chunks take the sliding pass only");
         any = true;
         bool ok = false;
         choice_cost(len, P->fh[f].T[axis], ch, &ok);
@@ -1855,6 +1930,7 @@ int fb_convolve_grid(int tail, int precision, int dim, const int64_t *size, cons
         tmp.device = device;
         tmp.nfields = 1;
         tmp.dim = dim;
+        tmp.inplace_passes = true;
         tmp.fh.assign(1, F);
         for (int a = 0; a < dim; ++a) {
             std::vector<PassWork> work;

@@ -40,11 +40,14 @@ struct FieldDev {
 };
 
 // One workgroup of a fast line pass: `nlines` lines of one field whose first
-// element offsets (within the field) are base + l*line_stride.
+// element offsets (within the field) are base + l*line_stride; for a field whose lines run in
+// chunks (PassField::chunk), chunk `chunk` of each.
 struct PassWork {
     int32_t field;
     int32_t nlines;
     int64_t base;
+    int32_t chunk = 0;
+    int32_t pad_ = 0;
 };
 
 // Doubles per line in the LDS of the prefix pass: positions -T-1 .. lpl*seg+T+1, rounded up
@@ -60,7 +63,7 @@ enum PassKind { kPassSlide = 0, kPassPrefix = 1 };
 
 // Per-field geometry of one fast line pass along one axis.
 struct PassField {
-    int64_t len;           // line length L
+    int64_t len;           // line length L the workgroup's LDS is laid out for (a chunk's, when chunked)
     int64_t line_stride;   // offset between consecutive lines of a workgroup
     int64_t elem_stride;   // offset between consecutive elements of a line
     int32_t T;
@@ -81,6 +84,13 @@ struct PassField {
     // kBlockedOut) and the y pass reads them so (kBlockedIn).  Every other pass input and
     // output is row-major.
     int32_t blocked;
+    // Lines longer than the pass takes run in chunks: chunk c stores nodes [c*chunk,
+    // (c+1)*chunk) of the line (clipped to full_len) and loads `halo` = num_iter*(T+1) more on
+    // each side within the line, since an output after num_iter iterations depends only on
+    // inputs that near.  chunk = 0: whole lines (len = full_len).
+    int32_t chunk;
+    int64_t full_len;
+    int32_t halo;
     int32_t pad_;
 };
 

@@ -977,8 +977,10 @@ __device__ __forceinline__ void inject_line(const FieldDev &F, int64_t line_in_f
                                             int fsel, double off, const uint32_t *__restrict__ vals,
                                             const uint32_t *__restrict__ line_start,
                                             const uint32_t *__restrict__ line_count, const double *__restrict__ pts,
-                                            const double *__restrict__ val, float *row)
+                                            const double *__restrict__ val, float *row, int lo, int len)
 {
+    // row[x - lo] receives cell x of the line for x in [lo, lo + len) (a chunk of a long line);
+    // cells outside are skipped
     const int nsub = 2 * source_rows(F.dim);
     uint32_t rsq[8], rcq[8], pre[9];
     line_streams(F, line_in_field, injector, line_start, line_count, rsq, rcq);
@@ -1023,9 +1025,10 @@ __device__ __forceinline__ void inject_line(const FieldDev &F, int64_t line_in_f
                 if (sid + d < iw && orid == rid) c += oc;
             }
         const bool head = valid && start;
-        float *cell = row + (x < 0 ? 0 : x);
+        const bool here = head && x >= lo && x < lo + len;
+        float *cell = row + (here ? x - lo : 0);
         for (int q = 0; q < nsub; ++q)
-            if (head && hq == q) *cell = (float)((double)*cell + c);
+            if (here && hq == q) *cell = (float)((double)*cell + c);
     }
 }
 
@@ -1059,14 +1062,27 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     const int half = nthr >> 1;
     const int fsel = tid >= half;            // 0: v worker, 1: w worker
     const int ltid = tid - fsel * half;
-    const int L = (int)P.len;
+    // line range of this work item: the whole line, or for a chunked field the chunk's nodes
+    // [lo + so, lo + so + sc) with their halos, [lo, lo + L); the LDS is laid out for P.len
+    int L = (int)P.len;
+    int lo = 0, so = 0, sc = L;
+    if (P.chunk) {
+        const int64_t c0 = (int64_t)wk.chunk * P.chunk;
+        const int64_t c1 = min(c0 + (int64_t)P.chunk, P.full_len);
+        const int64_t a0 = max(c0 - (int64_t)P.halo, (int64_t)0);
+        lo = (int)a0;
+        L = (int)(min(c1 + (int64_t)P.halo, P.full_len) - a0);
+        so = (int)(c0 - a0);
+        sc = (int)(c1 - c0);
+    }
     const int T = P.T;
     const int lpl = P.lpl;
     const int lines_cap = P.lines;
     const int pad = T + 1;
     const int rpad = P.rpad;
-    const int lstride = L + pad + rpad;
+    const int lstride = (int)P.len + pad + rpad;
     const int nl = wk.nlines;
+    const int64_t es = P.elem_stride;
     float *sv = reinterpret_cast<float *>(smem);
     float *sw = sv + (size_t)lines_cap * lstride;
     float *mine = fsel ? sw : sv;
@@ -1114,14 +1130,14 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
             }
         }
         if (adj_g)
-            adjoint_load(out + F.out_off + wk.base, wsave + F.out_off + wk.base, adj_g + F.out_off + wk.base, off,
-                         P.line_stride, nl, L, sv + pad, sw + pad, lstride);
+            adjoint_load(out + F.out_off + wk.base + lo, wsave + F.out_off + wk.base + lo,
+                         adj_g + F.out_off + wk.base + lo, off, P.line_stride, nl, L, sv + pad, sw + pad, lstride);
         else if (P.blocked == kBlockedIn)
             blocked_load_cols(gv + F.cell_off, gw + F.cell_off, F.size[0], wk.base, nl, L, sv + pad, sw + pad,
                               lstride);
         else {
-            tile_copy<true>(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
-            tile_copy<true>(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
+            tile_copy<true>(gv + F.cell_off + wk.base + lo * es, P.line_stride, es, nl, L, sv + pad, lstride);
+            tile_copy<true>(gw + F.cell_off + wk.base + lo * es, P.line_stride, es, nl, L, sw + pad, lstride);
         }
         __syncthreads();
     }
@@ -1129,7 +1145,7 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     if (INJECT) {
         // a line's injection writes cells 0..L-1 of that line only
         inject_line(F, line_in_field, injector, iw, sid, fsel, off, vals, line_start, line_count, pts, val,
-                    mine + my_line * lstride + pad);
+                    mine + my_line * lstride + pad, lo, L);
         line_sync(lpl);
     }
 
@@ -1182,10 +1198,12 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
 
     if (final_pass) {
         // masked quotient + offset (interpolation.py:367, :428-430), formed as the v lines are stored
-        tile_copy<false>(out + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride,
+        tile_copy<false>(out + F.out_off + wk.base + (lo + so) * es, P.line_stride, es, nl, sc, sv + pad + so, lstride,
                          LdsQuotient{(int)(sw - sv), off, F.thr_fast_f});
         // training forward: keep the normalised weight field for the adjoint
-        if (wsave) tile_copy<false>(wsave + F.out_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
+        if (wsave)
+            tile_copy<false>(wsave + F.out_off + wk.base + (lo + so) * es, P.line_stride, es, nl, sc, sw + pad + so,
+                             lstride);
     } else if (P.blocked == kBlockedOut) {
         const int64_t y0 = wk.base / F.size[0];
         blocked_store_rows(gv + F.cell_off, F.size[0], y0, nl, L, sv + pad, lstride);
@@ -1193,8 +1211,10 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     } else if (P.blocked == kBlockedIn) {
         blocked_store_cols(gv + F.cell_off, gw + F.cell_off, F.size[0], wk.base, nl, L, sv + pad, sw + pad, lstride);
     } else {
-        tile_copy<false>(gv + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sv + pad, lstride);
-        tile_copy<false>(gw + F.cell_off + wk.base, P.line_stride, P.elem_stride, nl, L, sw + pad, lstride);
+        tile_copy<false>(gv + F.cell_off + wk.base + (lo + so) * es, P.line_stride, es, nl, sc, sv + pad + so,
+                         lstride);
+        tile_copy<false>(gw + F.cell_off + wk.base + (lo + so) * es, P.line_stride, es, nl, sc, sw + pad + so,
+                         lstride);
     }
 }
 
@@ -1383,7 +1403,7 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_prefix_pass(
         const int iw = lpl < 64 ? lpl : 64;
         const int64_t line_in_field = wk.base / F.size[0] + my_line;
         inject_line(F, line_in_field, line_ok && sid < iw, iw, sid, fsel, off, vals, line_start, line_count, pts, val,
-                    myX);
+                    myX, 0, L);
     } else {
         if (adj_g)
             adjoint_load(out + F.out_off + wk.base, wsave + F.out_off + wk.base, adj_g + F.out_off + wk.base, off,

@@ -12,7 +12,10 @@ passes (marked `gpu`; through the C ABI, against the oracle).
 * the batch of 32 3D fields of 256x256x64 with N~logU[1e4,2e5]: FP64 and FP32C against the
   oracle on a sample of fields;
 * a plan destroyed right after an asynchronous execute: its buffers return to the pool only
-  after its kernels finish.
+  after its kernels finish;
+* FP32C lines longer than one line pass takes (x lines of 16384-40000 nodes in 1D/2D/3D, y lines
+  of 16384, a half kernel of 172), run in chunks with num_iter*(T+1) halos, against the oracle;
+  chunks along the middle axis of a 3D grid are refused.
 """
 import math
 
@@ -233,3 +236,58 @@ def test_destroy_after_async_execute():
         np.testing.assert_array_equal(np.isnan(out), np.isnan(ref))
         ok = ~np.isnan(ref)
         np.testing.assert_array_equal(out[ok].view(np.uint32), ref[ok].view(np.uint32))
+
+
+# ---------------------------------------------------------------------------
+# FP32C lines longer than one line pass takes: chunks with num_iter*(T+1) halos
+
+@pytest.mark.parametrize('W,H,transpose', [(20000, 24, False), (30011, 12, False), (16384, 20, True)])
+def test_fp32c_chunked_lines(orc, W, H, transpose):
+    """x lines (first pass, injecting) or y lines (last pass) beyond the 256-lane x 49-element
+    limit run as chunks; stored nodes depend only on inputs within the halo, so the result
+    meets the FP32C bar against the oracle everywhere, chunk seams included."""
+    c = _long_line_case(W, H, W + H, transpose)
+    out = _run(c, precision='fp32c')
+    ref = _check_fast(orc, c, out, 'chunked lines %s' % (c['size'],))
+    assert np.isfinite(ref).mean() > 0.5
+
+
+def test_fp32c_chunked_lines_wide_kernel(orc):
+    """A large sigma (half kernel 200, halo 4*201 = 804 per side) on 14000-node lines."""
+    rng = np.random.default_rng(5)
+    W, H = 14000, 12
+    step = 0.01
+    ext = np.array([W - 1, H - 1]) * step
+    n = 3000
+    pts = rng.uniform(0.0, 1.0, (n, 2)) * ext
+    val = 3.0 + np.sin(pts[:, 0]) + rng.normal(0.0, 0.2, n)
+    c = dict(pts=pts, val=val, sigma=[2.0, 0.04], x0=np.zeros(2), step=step, size=(W, H), num_iter=4,
+             max_dist=3.5)
+    out = _run(c, precision='fp32c')
+    _check_fast(orc, c, out, 'chunked wide kernel')
+
+
+def test_fp32c_chunked_1d_and_3d(orc):
+    rng = np.random.default_rng(6)
+    # 1D: one 40000-node line (first and last pass at once)
+    L = 40000
+    pts = rng.uniform(0.0, (L - 1) * 0.01, (6000, 1))
+    val = np.cos(pts[:, 0]) + rng.normal(0.0, 0.1, len(pts))
+    c1 = dict(pts=pts, val=val, sigma=0.05, x0=np.zeros(1), step=0.01, size=(L,), num_iter=4, max_dist=3.5)
+    _check_fast(orc, c1, _run(c1, precision='fp32c'), 'chunked 1D')
+    # 3D: long x lines (the injecting pass)
+    size = (13000, 7, 8)
+    ext = (np.array(size) - 1) * 0.01
+    pts = rng.uniform(0.0, 1.0, (20000, 3)) * ext
+    val = np.sin(pts[:, 0] * 3) + pts[:, 1] + rng.normal(0.0, 0.1, len(pts))
+    c3 = dict(pts=pts, val=val, sigma=0.03, x0=np.zeros(3), step=0.01, size=size, num_iter=3, max_dist=3.5)
+    _check_fast(orc, c3, _run(c3, precision='fp32c'), 'chunked 3D x')
+
+
+def test_fp32c_chunked_limits():
+    """Chunked lines along the middle axis of a 3D grid would be filtered in place and are refused
+    with a clear message (precision fp64 takes them)."""
+    c = dict(pts=np.zeros((4, 3)) + 0.05, val=np.ones(4), sigma=0.03, x0=np.zeros(3), step=0.01,
+             size=(8, 13000, 6), num_iter=3, max_dist=3.5)
+    with pytest.raises(RuntimeError, match='fp64'):
+        _run(c, precision='fp32c')
