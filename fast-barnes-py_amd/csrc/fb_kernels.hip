@@ -38,6 +38,32 @@ __device__ __forceinline__ int find_seg(const int64_t *__restrict__ prefix, int 
 // value offsets: (min + max)/2 over all values of a field, NaN if any value is NaN
 // (numpy's amin/amax propagate NaN), interpolation.py:209
 
+// min, max and a NaN flag of v[k] for k = k0, k0 + kBlock, ... < k1, the loads of eight steps
+// issued before their first use (one thread's share of a workgroup's values)
+__device__ __forceinline__ void minmax_range(const double *__restrict__ v, int64_t k0, int64_t k1, double &mn,
+                                             double &mx, int &isn)
+{
+    constexpr int kUnroll = 8;
+    int64_t k = k0;
+    for (; k + (kUnroll - 1) * kBlock < k1; k += kUnroll * kBlock) {
+        double x[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) x[u] = v[k + u * kBlock];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            if (x[u] != x[u]) isn = 1;
+            mn = fmin(mn, x[u]);
+            mx = fmax(mx, x[u]);
+        }
+    }
+    for (; k < k1; k += kBlock) {
+        const double x = v[k];
+        if (x != x) isn = 1;
+        mn = fmin(mn, x);
+        mx = fmax(mx, x);
+    }
+}
+
 __device__ __forceinline__ void offsets_block(const FieldDev *__restrict__ fields, const double *__restrict__ val,
                                               double *__restrict__ offsets, int f)
 {
@@ -48,12 +74,7 @@ __device__ __forceinline__ void offsets_block(const FieldDev *__restrict__ field
     if (tid == 0) snan = 0;
     double mn = INFINITY, mx = -INFINITY;
     int isn = 0;
-    for (int64_t k = tid; k < F.n; k += kBlock) {
-        const double v = val[F.pt_off + k];
-        if (v != v) isn = 1;
-        mn = fmin(mn, v);
-        mx = fmax(mx, v);
-    }
+    minmax_range(val + F.pt_off, tid, F.n, mn, mx, isn);
     smin[tid] = mn;
     smax[tid] = mx;
     __syncthreads();
@@ -91,12 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_offsets_part(const FieldDev *__restr
     const int64_t b1 = min(F.n, b0 + kOffChunk);
     double mn = INFINITY, mx = -INFINITY;
     int isn = 0;
-    for (int64_t k = b0 + tid; k < b1; k += kBlock) {
-        const double v = val[F.pt_off + k];
-        if (v != v) isn = 1;
-        mn = fmin(mn, v);
-        mx = fmax(mx, v);
-    }
+    minmax_range(val + F.pt_off, b0 + tid, b1, mn, mx, isn);
     smin[tid] = mn;
     smax[tid] = mx;
     __syncthreads();
@@ -274,13 +290,19 @@ __global__ __launch_bounds__(kBlock) void k_point_keys_hist(const FieldDev *__re
     __syncthreads();
     int rr[kKeysMaxPpt];
     uint32_t lr[kKeysMaxPpt];
+    // a thread's points ascend: its field is found once and then advanced
+    int f = p0 + threadIdx.x < p1 ? find_seg(pt_prefix, nfields, p0 + threadIdx.x) : 0;
+    int64_t fnext = f + 1 < nfields ? pt_prefix[f + 1] : INT64_MAX;
 #pragma unroll
     for (int k = 0; k < kKeysMaxPpt; ++k) {
         rr[k] = -1;
         lr[k] = 0u;
         const int64_t i = p0 + (int64_t)k * kBlock + threadIdx.x;
         if (k >= ppt || i >= p1) continue;
-        const int f = find_seg(pt_prefix, nfields, i);
+        while (i >= fnext) {
+            ++f;
+            fnext = f + 1 < nfields ? pt_prefix[f + 1] : INT64_MAX;
+        }
         const FieldDev &F = fields[f];
         PointPos pp;
         const bool inside = point_pos(F, pts + i * F.dim, pp);
@@ -1699,11 +1721,12 @@ int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int 
 {
     if (npoints <= 0) return FB_OK;
     // points per thread: enough for each workgroup to see many points per row, while keeping
-    // at least ~512 workgroups; small point sets, and fields of more rows than half the LDS
-    // counters (whose rows then rarely collide), take one atomic per point
+    // at least ~512 workgroups; small point sets take one atomic per point (workgroups of
+    // k_point_keys_hist whose rows exceed the LDS counters do too, for several points)
     const int64_t q = npoints / (512 * (int64_t)kBlock);
     const int64_t ppt = q < kKeysMaxPpt ? q : kKeysMaxPpt;
-    if (ppt >= 4 && d_row_count && max_rows <= kKeysBins / 2) {
+    (void)max_rows;
+    if (ppt >= 4 && d_row_count) {
         const int64_t per = ppt * kBlock;
         hipLaunchKernelGGL(k_point_keys_hist, dim3((unsigned)((npoints + per - 1) / per)), dim3(kBlock), 0, s,
                            d_fields, d_pt_prefix, nfields, npoints, (int)ppt, d_pts, d_keys, d_vals, d_row_count,
