@@ -420,6 +420,7 @@ struct fb_plan {
     uint32_t *d_keys = nullptr, *d_vals = nullptr, *d_keys2 = nullptr, *d_vals2 = nullptr;
     uint32_t *d_line_count = nullptr, *d_line_start = nullptr, *d_scan_tmp = nullptr;
     uint32_t *d_skeys = nullptr, *d_big_list = nullptr, *d_big_count = nullptr;   // row buckets
+    PointRec *d_rec = nullptr;                  // FP32C: each point's position for the injection
     double *d_vg = nullptr, *d_wg = nullptr, *d_vs = nullptr, *d_ws = nullptr;
     float *d_v = nullptr, *d_w = nullptr;
     PassWork *d_work[3] = {nullptr, nullptr, nullptr};
@@ -778,7 +779,7 @@ int run_pass(fb_plan *P, int axis, bool inject, const uint32_t *vals, const doub
 {
     for (const PassGroup &g : P->groups[axis]) {
         int rc = launch_fast_pass(g.kind, inject, g.seg, P->d_fields, P->d_offsets, P->d_work[axis] + g.first, g.count,
-                                  P->d_pass[axis], 2 * g.lpf, g.lds, nullptr, vals, P->d_line_start,
+                                  P->d_pass[axis], 2 * g.lpf, g.lds, P->d_rec, vals, P->d_line_start,
                                   P->d_line_count, pts, val, P->d_v, P->d_w, out, wsave, final_pass, adj_g, s);
         if (rc) return rc;
     }
@@ -884,6 +885,7 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
     } else {
         TRY(P->alloc(&P->d_v, P->ncells));
         TRY(P->alloc(&P->d_w, P->ncells));
+        TRY(P->alloc(&P->d_rec, P->npoints));
         TRY(build_passes(P));
     }
 #undef TRY
@@ -917,12 +919,12 @@ int plan_sort(fb_plan *P, const double *pts, const double *val, hipStream_t s, c
     if (offsets_single_pass(P->max_n, P->d_off_part)) {
         if ((rc = launch_point_keys_offsets(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints, pts, P->d_keys,
                                             P->d_vals, P->d_line_count, local ? nullptr : P->d_vals,
-                                            local ? nullptr : P->d_keys2, val, P->d_offsets, s)))
+                                            local ? nullptr : P->d_keys2, val, P->d_offsets, P->d_rec, s)))
             return rc;
     } else {
         if ((rc = launch_offsets(P->d_fields, P->nfields, P->max_n, val, P->d_off_part, P->d_offsets, s))) return rc;
         if ((rc = launch_point_keys(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints, pts, P->d_keys, P->d_vals,
-                                    P->d_line_count, local ? nullptr : P->d_vals, local ? nullptr : P->d_keys2, P->max_rows, s)))
+                                    P->d_line_count, local ? nullptr : P->d_vals, local ? nullptr : P->d_keys2, P->max_rows, P->d_rec, s)))
             return rc;
     }
     uint32_t *vs = P->d_vals2;
@@ -1244,7 +1246,7 @@ static int time_pass_subset(fb_plan *P, int a, const std::vector<PassWork> &work
         if (rep == 1) (void)hipEventRecord(e0, s);
         for (const PassGroup &g : fg) {
             rc = launch_fast_pass(g.kind, a == 0, g.seg, P->d_fields, P->d_offsets, dw + g.first, g.count, dp, 2 * g.lpf,
-                                  g.lds, nullptr, P->sorted_vals, P->d_line_start, P->d_line_count, pts, val, P->d_v,
+                                  g.lds, P->d_rec, P->sorted_vals, P->d_line_start, P->d_line_count, pts, val, P->d_v,
                                   P->d_w, out, nullptr, fin, nullptr, s);
             if (rc) break;
         }

@@ -108,6 +108,14 @@ __host__ __device__ inline int64_t blocked_index(int64_t y, int64_t x, int64_t W
     return (((y >> 3) * (W >> 2) + (x >> 2)) << 5) + ((y & 7) << 2) + (x & 3);
 }
 
+// A point's grid position as the FP32C injection uses it, formed once per point by the
+// point-key kernels (point_pos): its base cell along x and its fractional offsets per axis.
+struct PointRec {
+    double w[3];
+    int32_t xi;
+    int32_t pad_;
+};
+
 // Contribution record of the sorted injection list.
 struct Contrib {
     uint32_t key;          // global cell index (kInvalidKey when outside)

@@ -196,7 +196,8 @@ __device__ __forceinline__ void point_keys_block(const FieldDev *__restrict__ fi
                                                  const int64_t *__restrict__ pt_prefix, int nfields, int64_t npoints,
                                                  const double *__restrict__ pts, uint32_t *__restrict__ keys,
                                                  uint32_t *__restrict__ vals, uint32_t *__restrict__ row_count,
-                                                 uint32_t *__restrict__ slot, uint32_t *__restrict__ rowg, int64_t blk)
+                                                 uint32_t *__restrict__ slot, uint32_t *__restrict__ rowg,
+                                                 PointRec *__restrict__ rec, int64_t blk)
 {
     // row_count (may be null): per-row point counts (integer atomics).  slot/rowg (row buckets): the point's
     // position among its row's points as the atomics handed them out, and its global row
@@ -215,6 +216,7 @@ __device__ __forceinline__ void point_keys_block(const FieldDev *__restrict__ fi
     }
     const int64_t row = pp.i[2] * F.size[1] + pp.i[1];
     keys[i] = (uint32_t)(F.cell_off + row * F.size[0] + pp.i[0]);
+    if (rec) rec[i] = PointRec{{pp.w[0], pp.w[1], pp.w[2]}, (int32_t)pp.i[0], 0};
     if (row_count) {
         const uint32_t sl = atomicAdd(&row_count[F.xline_off + row], 1u);
         if (slot) {
@@ -229,9 +231,10 @@ __global__ __launch_bounds__(kBlock) void k_point_keys(const FieldDev *__restric
                                                        int64_t npoints, const double *__restrict__ pts,
                                                        uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                                        uint32_t *__restrict__ row_count,
-                                                       uint32_t *__restrict__ slot, uint32_t *__restrict__ rowg)
+                                                       uint32_t *__restrict__ slot, uint32_t *__restrict__ rowg,
+                                                       PointRec *__restrict__ rec)
 {
-    point_keys_block(fields, pt_prefix, nfields, npoints, pts, keys, vals, row_count, slot, rowg, blockIdx.x);
+    point_keys_block(fields, pt_prefix, nfields, npoints, pts, keys, vals, row_count, slot, rowg, rec, blockIdx.x);
 }
 
 // k_point_keys and k_offsets in one launch (they are independent): the first nkb workgroups
@@ -243,12 +246,14 @@ __global__ __launch_bounds__(kBlock) void k_point_keys_offsets(const FieldDev *_
                                                                uint32_t *__restrict__ vals,
                                                                uint32_t *__restrict__ row_count,
                                                                uint32_t *__restrict__ slot,
-                                                               uint32_t *__restrict__ rowg, int64_t nkb,
+                                                               uint32_t *__restrict__ rowg,
+                                                               PointRec *__restrict__ rec, int64_t nkb,
                                                                const double *__restrict__ val,
                                                                double *__restrict__ offsets)
 {
     if ((int64_t)blockIdx.x < nkb)
-        point_keys_block(fields, pt_prefix, nfields, npoints, pts, keys, vals, row_count, slot, rowg, blockIdx.x);
+        point_keys_block(fields, pt_prefix, nfields, npoints, pts, keys, vals, row_count, slot, rowg, rec,
+                         blockIdx.x);
     else
         offsets_block(fields, val, offsets, (int)(blockIdx.x - nkb));
 }
@@ -268,7 +273,8 @@ __global__ __launch_bounds__(kBlock) void k_point_keys_hist(const FieldDev *__re
                                                             int64_t npoints, int ppt, const double *__restrict__ pts,
                                                             uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                                             uint32_t *__restrict__ row_count,
-                                                            uint32_t *__restrict__ slot, uint32_t *__restrict__ rowg)
+                                                            uint32_t *__restrict__ slot, uint32_t *__restrict__ rowg,
+                                                            PointRec *__restrict__ rec)
 {
     __shared__ uint32_t hist[kKeysBins];
     __shared__ int64_t s_rbase;
@@ -314,6 +320,7 @@ __global__ __launch_bounds__(kBlock) void k_point_keys_hist(const FieldDev *__re
         }
         const int64_t row = pp.i[2] * F.size[1] + pp.i[1];
         keys[i] = (uint32_t)(F.cell_off + row * F.size[0] + pp.i[0]);
+        if (rec) rec[i] = PointRec{{pp.w[0], pp.w[1], pp.w[2]}, (int32_t)pp.i[0], 0};
         const int64_t grow = F.xline_off + row;
         if (span >= 0) {
             rr[k] = (int)(grow - rbase);
@@ -986,11 +993,20 @@ __device__ __forceinline__ void adjoint_load(const float *__restrict__ f, const 
     }
 }
 
+// Lane i receives lane i+1's value (from_next) or lane i-1's (from_prev) by a DPP wave shift;
+// the lane without a source keeps its own value.
+__device__ __forceinline__ int dpp_from_next(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_from_prev(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ double dpp_from_next(double v)
+{
+    return __hiloint2double(dpp_from_next(__double2hiint(v)), dpp_from_next(__double2loint(v)));
+}
+
 // Deterministic fused injection of one line's points into its float32 row (element x at
 // row[x]).  The line's sub-streams (line_streams: one base row, one corner column),
 // concatenated, are taken in chunks of up to 64 entries by the line's first `iw` lanes
-// (`injector`).  A segmented suffix sum over adjacent lanes of equal target cell (fixed
-// tree order) gives each run's total to its first lane.  Equal cells adjacent in one
+// (`injector`).  A run sum over adjacent lanes of equal target cell (in lane order) gives
+// each run's total to its first lane.  Equal cells adjacent in one
 // sub-stream form one run; a cell can head two runs of a chunk only from different
 // sub-streams, so the heads add their totals in one round per sub-stream of the head
 // (program order), without conflicts.  The result does not depend on timing.  Called by
@@ -998,7 +1014,7 @@ __device__ __forceinline__ void adjoint_load(const float *__restrict__ f, const 
 __device__ __forceinline__ void inject_line(const FieldDev &F, int64_t line_in_field, bool injector, int iw, int sid,
                                             int fsel, double off, const uint32_t *__restrict__ vals,
                                             const uint32_t *__restrict__ line_start,
-                                            const uint32_t *__restrict__ line_count, const double *__restrict__ pts,
+                                            const uint32_t *__restrict__ line_count, const PointRec *__restrict__ rec,
                                             const double *__restrict__ val, float *row, int lo, int len)
 {
     // row[x - lo] receives cell x of the line for x in [lo, lo + len) (a chunk of a long line);
@@ -1011,46 +1027,95 @@ __device__ __forceinline__ void inject_line(const FieldDev &F, int64_t line_in_f
     for (int q = 0; q < 8; ++q) pre[q + 1] = pre[q] + rcq[q];
     int nch = (int)((pre[8] + iw - 1) / iw);
     for (int d = 32; d >= 1; d >>= 1) nch = max(nch, __shfl_xor(nch, d));
-    for (int ch = 0; ch < nch; ++ch) {
+    // This lane's entry of chunk ch: its sub-stream and its slot in the sorted point list.
+    auto entry = [&](int ch, int &hq, uint32_t &src) {
         const uint32_t e = (uint32_t)ch * iw + sid;
-        const bool valid = injector && e < pre[8];
-        int x = -1, hq = 0;
-        double c = 0.0;
-        if (valid) {
-            uint32_t src = 0;
+        hq = 0;
+        src = 0;
+        if (!injector || e >= pre[8]) return false;
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
-                if (e >= pre[q] && e < pre[q + 1]) {
-                    hq = q;
-                    src = rsq[q] + (e - pre[q]);
-                }
-            const uint32_t p = vals[src];
-            PointPos pp;
-            point_pos(F, pts + (int64_t)p * F.dim, pp);
-            x = (int)pp.i[0] + (hq & 1);
-            const double w = corner_weight(F.dim, pp, hq & 1, (hq >> 1) & 1, hq >> 2);
-            c = fsel ? w : w * (val[p] - off);
-        }
-        // runs: maximal groups of adjacent lanes with equal cell; run id = inclusive count
-        // of run starts (the concatenated stream is not sorted, so equal cells further
-        // apart belong to different runs)
-        const int px = __shfl_up(x, 1, iw);
-        const bool start = sid == 0 || px != x;
-        const int lane = threadIdx.x & 63;
-        const int rid = __popcll(__ballot(start) & ((2ull << lane) - 1ull));
-        // the segmented sum only where some run of this wave's chunk has more than one entry
-        // (with sparse stations, rarely): otherwise every lane is its own run and keeps c
-        if (__ballot(valid && !start))
-            for (int d = 1; d < iw; d <<= 1) {
-                const double oc = __shfl_down(c, d, iw);
-                const int orid = __shfl_down(rid, d, iw);
-                if (sid + d < iw && orid == rid) c += oc;
+        for (int q = 0; q < 8; ++q)
+            if (e >= pre[q] && e < pre[q + 1]) {
+                hq = q;
+                src = rsq[q] + (e - pre[q]);
             }
-        const bool head = valid && start;
-        const bool here = head && x >= lo && x < lo + len;
-        float *cell = row + (here ? x - lo : 0);
-        for (int q = 0; q < nsub; ++q)
-            if (here && hq == q) *cell = (float)((double)*cell + c);
+        return true;
+    };
+    struct PointData {
+        PointRec r;
+        double v;
+    };
+    auto load_point = [&](bool ok, uint32_t p, PointData &o) {
+        o.r = PointRec{{0.0, 0.0, 0.0}, 0, 0};
+        o.v = 0.0;
+        if (!ok) return;
+        o.r = rec[p];
+        o.v = val[p];
+    };
+    // Software pipeline over the chunks: while chunk ch is summed, the point data of chunk ch+1
+    // and the point index of chunk ch+2 are in flight.
+    auto fetch_index = [&](int ch, bool &ok, int &hq, uint32_t &p) {
+        uint32_t src;
+        ok = entry(ch, hq, src);
+        p = ok ? vals[src] : 0u;
+    };
+    bool ok_a, ok_b;
+    int hq_a, hq_b;
+    uint32_t p_a, p_b;
+    PointData pd_a;
+    fetch_index(0, ok_a, hq_a, p_a);
+    load_point(ok_a, p_a, pd_a);
+    fetch_index(1, ok_b, hq_b, p_b);
+    for (int ch = 0; ch < nch; ++ch) {
+        const bool valid = ok_a;
+        const int hq = hq_a;
+        const PointData pd = pd_a;
+        ok_a = ok_b;
+        hq_a = hq_b;
+        load_point(ok_b, p_b, pd_a);
+        fetch_index(ch + 2, ok_b, hq_b, p_b);
+        {
+            int x = -1;
+            double c = 0.0;
+            if (valid) {
+                PointPos pp;
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    pp.i[m] = 0;
+                    pp.w[m] = pd.r.w[m];
+                }
+                x = pd.r.xi + (hq & 1);
+                const double w = corner_weight(F.dim, pp, hq & 1, (hq >> 1) & 1, hq >> 2);
+                c = fsel ? w : w * (pd.v - off);
+            }
+            // runs: maximal groups of adjacent lanes with equal cell; run id = inclusive count
+            // of run starts (the concatenated stream is not sorted, so equal cells further
+            // apart belong to different runs)
+            const int px = dpp_from_prev(x);
+            const bool start = sid == 0 || px != x;
+            const int lane = threadIdx.x & 63;
+            const int rid = __popcll(__ballot(start) & ((2ull << lane) - 1ull));
+            // The run sum only where some run of this wave's chunk has more than one entry:
+            // otherwise every lane is its own run and keeps c.  Each run's first lane adds the
+            // contributions of the run's next lanes one at a time (DPP wave shifts, no LDS
+            // round trips), in lane order; runs are short, so this stops after a few steps.
+            if (__ballot(valid && !start)) {
+                double o = c;
+                int orid = rid;
+                for (int t = 1; t < iw; ++t) {
+                    o = dpp_from_next(o);
+                    orid = dpp_from_next(orid);
+                    const bool take = sid + t < iw && orid == rid;
+                    if (take) c += o;
+                    if (!__ballot(take)) break;
+                }
+            }
+            const bool head = valid && start;
+            const bool here = head && x >= lo && x < lo + len;
+            float *cell = row + (here ? x - lo : 0);
+            for (int q = 0; q < nsub; ++q)
+                if (here && hq == q) *cell = (float)((double)*cell + c);
+        }
     }
 }
 
@@ -1069,7 +1134,7 @@ __device__ __forceinline__ void line_sync(int lpl)
 template <bool INJECT, int SEG>
 __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     const FieldDev *__restrict__ fields, const double *__restrict__ offsets, const PassWork *__restrict__ work,
-    const PassField *__restrict__ passes, const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
+    const PassField *__restrict__ passes, const PointRec *__restrict__ rec, const uint32_t *__restrict__ vals,
     const uint32_t *__restrict__ line_start, const uint32_t *__restrict__ line_count, const double *__restrict__ pts,
     const double *__restrict__ val, float *__restrict__ gv, float *__restrict__ gw, float *__restrict__ out,
     float *__restrict__ wsave, int final_pass, const float *__restrict__ adj_g)
@@ -1166,7 +1231,7 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
 
     if (INJECT) {
         // a line's injection writes cells 0..L-1 of that line only
-        inject_line(F, line_in_field, injector, iw, sid, fsel, off, vals, line_start, line_count, pts, val,
+        inject_line(F, line_in_field, injector, iw, sid, fsel, off, vals, line_start, line_count, rec, val,
                     mine + my_line * lstride + pad, lo, L);
         line_sync(lpl);
     }
@@ -1384,7 +1449,7 @@ __device__ __forceinline__ double prefix_step(const double *pl, const double *pr
 template <bool INJECT, int SEG>
 __global__ __launch_bounds__(kFastBlock, 1) void k_prefix_pass(
     const FieldDev *__restrict__ fields, const double *__restrict__ offsets, const PassWork *__restrict__ work,
-    const PassField *__restrict__ passes, const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
+    const PassField *__restrict__ passes, const PointRec *__restrict__ rec, const uint32_t *__restrict__ vals,
     const uint32_t *__restrict__ line_start, const uint32_t *__restrict__ line_count, const double *__restrict__ pts,
     const double *__restrict__ val, float *__restrict__ gv, float *__restrict__ gw, float *__restrict__ out,
     float *__restrict__ wsave, int final_pass, const float *__restrict__ adj_g)
@@ -1424,7 +1489,7 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_prefix_pass(
         __syncthreads();
         const int iw = lpl < 64 ? lpl : 64;
         const int64_t line_in_field = wk.base / F.size[0] + my_line;
-        inject_line(F, line_in_field, line_ok && sid < iw, iw, sid, fsel, off, vals, line_start, line_count, pts, val,
+        inject_line(F, line_in_field, line_ok && sid < iw, iw, sid, fsel, off, vals, line_start, line_count, rec, val,
                     myX, 0, L);
     } else {
         if (adj_g)
@@ -1704,20 +1769,20 @@ bool offsets_single_pass(int64_t max_n, const double *d_part) { return !d_part |
 int launch_point_keys_offsets(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
                               const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_row_count,
                               uint32_t *d_slot, uint32_t *d_rowg, const double *d_val, double *d_offsets,
-                              hipStream_t s)
+                              PointRec *d_rec, hipStream_t s)
 {
     if (nfields <= 0) return FB_OK;
     const int64_t nkb = npoints > 0 ? nblocks(npoints) : 0;
     hipLaunchKernelGGL(k_point_keys_offsets, dim3((unsigned)(nkb + nfields)), dim3(kBlock), 0, s, d_fields,
-                       d_pt_prefix, nfields, npoints, d_pts, d_keys, d_vals, d_row_count, d_slot, d_rowg, nkb, d_val,
-                       d_offsets);
+                       d_pt_prefix, nfields, npoints, d_pts, d_keys, d_vals, d_row_count, d_slot, d_rowg, d_rec, nkb,
+                       d_val, d_offsets);
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }
 
 int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
                       const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_row_count,
-                      uint32_t *d_slot, uint32_t *d_rowg, int64_t max_rows, hipStream_t s)
+                      uint32_t *d_slot, uint32_t *d_rowg, int64_t max_rows, PointRec *d_rec, hipStream_t s)
 {
     if (npoints <= 0) return FB_OK;
     // points per thread: enough for each workgroup to see many points per row, while keeping
@@ -1730,10 +1795,10 @@ int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int 
         const int64_t per = ppt * kBlock;
         hipLaunchKernelGGL(k_point_keys_hist, dim3((unsigned)((npoints + per - 1) / per)), dim3(kBlock), 0, s,
                            d_fields, d_pt_prefix, nfields, npoints, (int)ppt, d_pts, d_keys, d_vals, d_row_count,
-                           d_slot, d_rowg);
+                           d_slot, d_rowg, d_rec);
     } else {
         hipLaunchKernelGGL(k_point_keys, dim3(nblocks(npoints)), dim3(kBlock), 0, s, d_fields, d_pt_prefix, nfields,
-                           npoints, d_pts, d_keys, d_vals, d_row_count, d_slot, d_rowg);
+                           npoints, d_pts, d_keys, d_vals, d_row_count, d_slot, d_rowg, d_rec);
     }
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
@@ -1836,7 +1901,7 @@ bool fast_pass_seg_supported(int kind, int seg)
 
 int launch_fast_pass(int kind, bool inject, int seg, const FieldDev *d_fields, const double *d_offsets,
                      const PassWork *d_work, int nwork, const PassField *d_pass, int threads, size_t lds_bytes,
-                     const uint32_t *d_keys, const uint32_t *d_vals, const uint32_t *d_line_start,
+                     const PointRec *d_rec, const uint32_t *d_vals, const uint32_t *d_line_start,
                      const uint32_t *d_line_count, const double *d_pts, const double *d_val, float *d_v, float *d_w,
                      float *d_out, float *d_wsave, int final_pass, const float *d_adj_g, hipStream_t s)
 {
@@ -1853,11 +1918,11 @@ int launch_fast_pass(int kind, bool inject, int seg, const FieldDev *d_fields, c
     case S:                                                                                                        \
         if (inject)                                                                                                \
             hipLaunchKernelGGL((K<true, S>), dim3(nwork), dim3(threads), lds_bytes, s, d_fields, d_offsets, d_work, \
-                               d_pass, d_keys, d_vals, d_line_start, d_line_count, d_pts, d_val, d_v, d_w, d_out,   \
+                               d_pass, d_rec, d_vals, d_line_start, d_line_count, d_pts, d_val, d_v, d_w, d_out,    \
                                d_wsave, final_pass, d_adj_g);                                                      \
         else                                                                                                       \
             hipLaunchKernelGGL((K<false, S>), dim3(nwork), dim3(threads), lds_bytes, s, d_fields, d_offsets,        \
-                               d_work, d_pass, d_keys, d_vals, d_line_start, d_line_count, d_pts, d_val, d_v, d_w,  \
+                               d_work, d_pass, d_rec, d_vals, d_line_start, d_line_count, d_pts, d_val, d_v, d_w,   \
                                d_out, d_wsave, final_pass, d_adj_g);                                               \
         break;
 #define FB_LAUNCH_SLIDE(S) FB_LAUNCH(k_fast_pass, S)

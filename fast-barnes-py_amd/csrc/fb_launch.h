@@ -27,17 +27,17 @@ int launch_offsets(const FieldDev *d_fields, int nfields, int64_t max_n, const d
 // one (base cell, point) pair per point and, unless d_row_count is NULL, per-base-row point
 // counts (row = x-line index) by integer atomics; with d_slot/d_rowg (row buckets) each point's
 // slot in its row and its row instead of the point indices.  max_rows: the most x-lines of any
-// field (fields of at most 4096 count their points per row in LDS first)
+// field.  d_rec (may be NULL): each in-grid point's PointRec, for the FP32C injection
 int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
                       const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_row_count,
-                      uint32_t *d_slot, uint32_t *d_rowg, int64_t max_rows, hipStream_t stream);
+                      uint32_t *d_slot, uint32_t *d_rowg, int64_t max_rows, PointRec *d_rec, hipStream_t stream);
 // launch_point_keys and, when the offsets take one workgroup per field (offsets_single_pass),
 // launch_offsets in one launch
 bool offsets_single_pass(int64_t max_n, const double *d_part);
 int launch_point_keys_offsets(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
                               const double *d_pts, uint32_t *d_keys, uint32_t *d_vals, uint32_t *d_row_count,
                               uint32_t *d_slot, uint32_t *d_rowg, const double *d_val, double *d_offsets,
-                              hipStream_t stream);
+                              PointRec *d_rec, hipStream_t stream);
 // fb_sort.hip -- row buckets for fields of more than kLocalSortMax points: scatter the in-grid
 // points into their rows (d_row_start from the scan of the counts), then sort each row by
 // (key, point index); d_svals/d_skeys receive the sorted list, d_tkeys/d_tvals are scratch for
@@ -72,7 +72,7 @@ bool fast_pass_seg_supported(int kind, int seg);
 // kind: kPassSlide (k_fast_pass) or kPassPrefix (k_prefix_pass).  d_adj_g (first pass of the
 // adjoint): the tile load forms a = g/W, b = -a(f - offset) from d_out = f, d_wsave = W, d_adj_g = g.
 int launch_fast_pass(int kind, bool inject, int seg, const FieldDev *d_fields, const double *d_offsets, const PassWork *d_work,
-                     int nwork, const PassField *d_pass, int threads, size_t lds_bytes, const uint32_t *d_keys,
+                     int nwork, const PassField *d_pass, int threads, size_t lds_bytes, const PointRec *d_rec,
                      const uint32_t *d_vals, const uint32_t *d_line_start, const uint32_t *d_line_count,
                      const double *d_pts, const double *d_val, float *d_v, float *d_w, float *d_out,
                      float *d_wsave, int final_pass, const float *d_adj_g, hipStream_t stream);
