@@ -524,12 +524,29 @@ def main():
         log('naive subgrid')
         nv = barnes(c0['pts'], c0['val'], c0['sigma'], c0['x0'], c0['step'] * 10, (240, 120), method='naive')
         sub = f0[::10, ::10].astype(np.float64)
+        sub64 = ref[::10, ::10].astype(np.float64)
         okn = ~np.isnan(sub) & ~np.isnan(nv)
+        # interior: nodes at least max_dist*sigma from every border, where no station outside
+        # the grid (which naive Barnes uses and the convolution drops) has weight above the
+        # max_dist cut-off
+        W0, H0 = c0['size']
+        xs = np.arange(0, W0, 10) * c0['step']
+        ys = np.arange(0, H0, 10) * c0['step']
+        reach = 3.5 * c0['sigma']
+        inner = ((ys >= reach) & (ys <= (H0 - 1) * c0['step'] - reach))[:, None] & \
+                ((xs >= reach) & (xs <= (W0 - 1) * c0['step'] - reach))[None, :]
+        oki = okn & inner
+        okn64 = ~np.isnan(sub64) & ~np.isnan(nv)
         field_rate = args.fields / (ms_step * 1e-3)
         extras = dict(
             rmse_vs_fp64=float(np.sqrt(np.mean(d * d))), maxabs_vs_fp64=float(np.max(np.abs(d))),
             mask_mismatch_vs_fp64=int((np.isnan(ref) != np.isnan(f0)).sum()),
             rmse_vs_naive_subgrid=float(np.sqrt(np.mean((sub[okn] - nv[okn]) ** 2))),
+            rmse_vs_naive_subgrid_interior=float(np.sqrt(np.mean((sub[oki] - nv[oki]) ** 2))),
+            maxabs_vs_naive_subgrid_interior=float(np.max(np.abs(sub[oki] - nv[oki]))),
+            rmse_fp64_vs_naive_subgrid=float(np.sqrt(np.mean((sub64[okn64] - nv[okn64]) ** 2))),
+            rmse_fp64_vs_naive_subgrid_interior=float(np.sqrt(np.mean((sub64[okn64 & inner] - nv[okn64 & inner]) ** 2))),
+            naive_subgrid_interior_nodes=int(oki.sum()),
             fp64_dropin_ms_per_field=round(fp64_dropin_s * 1e3, 3),
             fp64_device_ms_per_field=round(fp64_dev_ms, 4),
             fp32c_batched_fields_per_s=round(field_rate, 1),

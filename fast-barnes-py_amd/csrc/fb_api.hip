@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdio>
 #include <limits>
 #include <cstring>
 #include <map>
@@ -1086,6 +1087,18 @@ int fb_version(void) { return 10000; }
 
 const char *fb_last_error(void) { return fb::last_error(); }
 
+int fb_device_arch(char *arch, int cap, int *compute_units)
+{
+    if (!arch || cap <= 0) return fail(FB_ERR_INVALID_ARGUMENT, "arch buffer is NULL or empty");
+    int dev = 0;
+    FB_HIP_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    FB_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+    std::snprintf(arch, (size_t)cap, "%s", prop.gcnArchName);
+    if (compute_units) *compute_units = prop.multiProcessorCount;
+    return FB_OK;
+}
+
 int fb_device_count(int *count)
 {
     if (!count) return fail(FB_ERR_INVALID_ARGUMENT, "count is NULL");
@@ -1376,6 +1389,33 @@ int fb_plan_choose_pass(fb_plan *P, int axis, int64_t len, int kind, int seg, in
     }
     if (P->gexec) { (void)hipGraphExecDestroy(P->gexec); P->gexec = nullptr; }
     if (P->graph) { (void)hipGraphDestroy(P->graph); P->graph = nullptr; }
+    return FB_OK;
+}
+
+int fb_plan_pass_choices(fb_plan *P, int axis, int64_t *len, int32_t *kind, int32_t *seg, int32_t *threads,
+                         int max_classes, int *nclasses, int32_t *field_class)
+{
+    if (!P || !nclasses) return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (P->precision != FB_PREC_FP32C) return fail(FB_ERR_INVALID_ARGUMENT, "pass choices apply to FP32C plans");
+    if (axis < 0 || axis >= P->dim) return fail(FB_ERR_INVALID_ARGUMENT, "axis out of range");
+    std::map<int64_t, int> cls;   // pass line length -> a field with it (the first)
+    for (int f = 0; f < P->nfields; ++f) cls.emplace(pass_len(P->fh[f], axis), f);
+    std::map<int64_t, int> index;
+    int i = 0;
+    for (const auto &c : cls) {
+        index[c.first] = i;
+        if (i < max_classes) {
+            const PassChoice ch = pass_choice(P, axis, c.first, P->fh[c.second].T[axis]);
+            if (len) len[i] = c.first;
+            if (kind) kind[i] = ch.kind;
+            if (seg) seg[i] = ch.seg;
+            if (threads) threads[i] = 2 * ch.lpf;
+        }
+        ++i;
+    }
+    *nclasses = i;
+    if (field_class)
+        for (int f = 0; f < P->nfields; ++f) field_class[f] = index[pass_len(P->fh[f], axis)];
     return FB_OK;
 }
 

@@ -59,6 +59,7 @@ SIGNATURES = {
     'fb_version': (ctypes.c_int, []),
     'fb_last_error': (ctypes.c_char_p, []),
     'fb_device_count': (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    'fb_device_arch': (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     'fb_half_kernel_size': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _i32p]),
     'fb_tail_value': (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp]),
     'fb_sigma_effective': (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp]),
@@ -85,6 +86,8 @@ SIGNATURES = {
     'fb_plan_autotune': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     'fb_plan_choose_pass': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int]),
+    'fb_plan_pass_choices': (ctypes.c_int, [_vp, ctypes.c_int, _i64p, _i32p, _i32p, _i32p, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_int), _i32p]),
     'fb_plan_pass_groups': (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _i32p, _i32p, ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_int)]),
     'fb_plan_set_timing': (ctypes.c_int, [_vp, ctypes.c_int]),

@@ -86,6 +86,7 @@ class BarnesBatch:
         self.total_points = pt
         self.total_out = out
         self._fields = arr
+        self._blocked = True
         self._plan = ctypes.c_void_p()
         check(lib().fb_plan_create(ctypes.byref(self._plan), arr, len(fields), self.prec, 0))
 
@@ -117,11 +118,83 @@ class BarnesBatch:
         check(lib().fb_plan_backward(self._plan, d_pts, d_val, d_out, d_wsave, d_gout, d_gval, d_gpts or None,
                                      stream))
 
-    def autotune(self, d_pts, d_val, d_out, stream):
+    def autotune(self, d_pts, d_val, d_out, stream, cache=None):
         """Times the pass kinds, segment lengths and workgroup sizes of every line pass on these
         buffers and keeps the fastest per axis and line length (FP32C plans).  d_out is left
-        undefined."""
+        undefined.
+
+        cache: a tunecache.TuneCache or the path of its JSON file.  When it holds every
+        line-length class of this plan, the stored choices are applied without timing and
+        'cache' is returned; otherwise the plan is timed, its choices are stored (the file is
+        rewritten) and 'tuned' is returned.  FP64 plans have nothing to tune ('none')."""
+        if self.prec != _lib.FB_PREC_FP32C:
+            check(lib().fb_plan_autotune(self._plan, d_pts, d_val, d_out, stream))
+            return 'none'
+        if cache is None:
+            check(lib().fb_plan_autotune(self._plan, d_pts, d_val, d_out, stream))
+            return 'tuned'
+        from fastbarnes import tunecache
+        tc = cache if isinstance(cache, tunecache.TuneCache) else tunecache.TuneCache(cache)
+        keys = self._class_keys()
+        hits = [(axis, length, tc.get(key)) for axis, length, key in keys]
+        if all(e is not None for _, _, e in hits):
+            try:
+                for axis, length, e in hits:
+                    self.choose_pass(axis, length, e['kind'], e['seg'], e['threads'])
+                return 'cache'
+            except RuntimeError:
+                pass   # an entry that no longer fits: time the plan afresh (overrides every class)
         check(lib().fb_plan_autotune(self._plan, d_pts, d_val, d_out, stream))
+        chosen = {}
+        for axis in range(self.dim):
+            for length, kind, seg, threads in self.pass_choices(axis)[0]:
+                chosen[(axis, length)] = (kind, seg, threads)
+        for axis, length, key in keys:
+            tc.put(key, *chosen[(axis, length)])
+        tc.save()
+        return 'tuned'
+
+    def pass_choices(self, axis):
+        """([(line length, kind, segment length, workgroup threads)] per line-length class of the
+        pass along `axis`, [class index of each field]) with the choices in force (autotuned,
+        forced, or the cost model's); FP32C plans."""
+        cap = 16
+        while True:
+            ln = (ctypes.c_int64 * cap)()
+            kd = (ctypes.c_int32 * cap)()
+            sg = (ctypes.c_int32 * cap)()
+            th = (ctypes.c_int32 * cap)()
+            fc = (ctypes.c_int32 * self.nfields)()
+            n = ctypes.c_int()
+            check(lib().fb_plan_pass_choices(self._plan, int(axis), ln, kd, sg, th, cap, ctypes.byref(n), fc))
+            if n.value <= cap:
+                break
+            cap = n.value
+        return ([(ln[i], ('slide', 'prefix')[kd[i]], sg[i], th[i]) for i in range(n.value)], list(fc))
+
+    def _class_keys(self):
+        """[(axis, line length, cache key)] of every line-length class of the plan."""
+        from fastbarnes import tunecache
+        arch = ctypes.create_string_buffer(256)
+        cus = ctypes.c_int()
+        check(lib().fb_device_arch(arch, 256, ctypes.byref(cus)))
+        dim = self.dim
+        halves = []
+        for a in self._fields:
+            sig = (ctypes.c_double * 3)(*a.sigma)
+            stp = (ctypes.c_double * 3)(*a.step)
+            T = (ctypes.c_int32 * 3)()
+            check(lib().fb_half_kernel_size(a.method, dim, sig, stp, a.num_iter, T))
+            halves.append(list(T))
+        out = []
+        for axis in range(dim):
+            classes, fclass = self.pass_choices(axis)
+            for ci, (length, _, _, _) in enumerate(classes):
+                members = [(tuple(int(s) for s in a.size[:dim]), halves[f][axis], a.num_iter, a.method)
+                           for f, a in enumerate(self._fields) if fclass[f] == ci]
+                out.append((axis, length, tunecache.class_key(arch.value.decode(), cus.value, self.precision, axis,
+                                                              length, self._blocked, members)))
+        return out
 
     def choose_pass(self, axis, length, kind, seg, threads=256):
         """Forces the pass along `axis` for lines of `length`: kind 'slide' or 'prefix', the
@@ -151,6 +224,7 @@ class BarnesBatch:
     def set_blocked(self, enable):
         """Allows (default) or forbids the blocked intermediate; the same results either way."""
         check(lib().fb_plan_set_blocked(self._plan, int(bool(enable))))
+        self._blocked = bool(enable)
 
     def set_timing(self, enable):
         check(lib().fb_plan_set_timing(self._plan, int(bool(enable))))

@@ -55,6 +55,11 @@ const char *fb_last_error(void);
 /* Number of visible HIP devices (0 when none).  Host-only. */
 int fb_device_count(int *count);
 
+/* Architecture name of the current device (hipDeviceProp_t::gcnArchName, e.g.
+ * "gfx950:sramecc+:xnack-", truncated to cap-1 characters) and its compute-unit count (may be
+ * NULL); a key for cached tuning results.  No reference counterpart. */
+int fb_device_arch(char *arch, int cap, int *compute_units);
+
 /* ---- kernel parameters (host-only, no GPU needed) ---------------------- */
 
 /* Half kernel size T per axis.
@@ -216,6 +221,17 @@ int fb_plan_autotune(fb_plan *plan, const double *d_pts, const double *d_val, fl
  * have length `len`, and rebuilds the plan's passes; for manual tuning and tests (no
  * reference counterpart).  FP32C plans only. */
 int fb_plan_choose_pass(fb_plan *plan, int axis, int64_t len, int kind, int seg, int threads);
+
+/* The line-length classes of the fast pass along `axis` of an FP32C plan, ascending by the
+ * length of the lines the pass runs (a chunk's length for lines that run in chunks), with the
+ * choice in force for each: kind (FB_PASS_SLIDE or FB_PASS_PREFIX), segment length and
+ * workgroup threads, autotuned or forced or else the cost model's.  *nclasses receives the
+ * number of classes; at most max_classes entries are written.  field_class (nfields entries,
+ * may be NULL) receives each field's class index.  Any output array may be NULL.  With
+ * fb_plan_choose_pass this saves and restores an autotuned plan's choices.  No reference
+ * counterpart. */
+int fb_plan_pass_choices(fb_plan *plan, int axis, int64_t *len, int32_t *kind, int32_t *seg, int32_t *threads,
+                         int max_classes, int *nclasses, int32_t *field_class);
 
 /* The launches of the fast pass along `axis`: segment length, workgroup size (threads), work
  * items and kind (FB_PASS_SLIDE or FB_PASS_PREFIX) of each; any output array may be NULL. */

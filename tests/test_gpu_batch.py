@@ -144,6 +144,50 @@ def test_autotune_keeps_results_and_reports_groups():
         _bitwise(d_out.numpy(), after)
 
 
+def test_autotune_cache_round_trip(tmp_path):
+    """A plan tuned with a cache file stores one entry per line-length class; a new plan of the
+    same batch takes every choice from the file without timing ('cache'), and runs the same
+    passes to the same bits.  Other stations on the same grids hit the same entries; an entry
+    that no longer fits (a segment length with no instantiation) makes the plan time afresh."""
+    import json
+    from fastbarnes import hip, workloads
+    from fastbarnes.batched import BarnesBatch
+    fs = [workloads.config3_field(i) for i in range(4)]
+    path = tmp_path / 'tune.json'
+
+    def run(fields, cache):
+        batch = BarnesBatch([_desc(c) for c in fields], precision='fp32c')
+        pts = np.concatenate([c['pts'] for c in fields])
+        val = np.concatenate([c['val'] for c in fields])
+        d_pts, d_val = hip.DeviceArray.from_numpy(pts), hip.DeviceArray.from_numpy(val)
+        d_out = hip.DeviceArray((batch.total_out,), np.float32)
+        s = hip.Stream()
+        how = batch.autotune(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr, cache=cache)
+        batch.execute(d_pts.ptr, d_val.ptr, d_out.ptr, s.ptr, graph=True)
+        s.synchronize()
+        return how, [batch.pass_choices(a)[0] for a in range(2)], d_out.numpy()
+
+    how1, ch1, out1 = run(fs, str(path))
+    assert how1 == 'tuned'
+    entries = json.loads(path.read_text())['entries']
+    assert len(entries) == sum(len(c) for c in ch1)
+    how2, ch2, out2 = run(fs, str(path))
+    assert how2 == 'cache' and ch2 == ch1
+    _bitwise(out2, out1)
+    # the same grids with other stations: the key has no point counts
+    fs3 = [dict(c, pts=c['pts'][: len(c['pts']) // 2], val=c['val'][: len(c['val']) // 2]) for c in fs]
+    how3, ch3, _ = run(fs3, str(path))
+    assert how3 == 'cache' and ch3 == ch1
+    # a stale entry: time afresh and rewrite it
+    d = json.loads(path.read_text())
+    k0 = sorted(d['entries'])[0]
+    d['entries'][k0] = dict(kind='slide', seg=7, threads=256)
+    path.write_text(json.dumps(d))
+    how4, ch4, _ = run(fs, str(path))
+    assert how4 == 'tuned'
+    assert json.loads(path.read_text())['entries'][k0]['seg'] != 7
+
+
 def test_full_size_batch_matches_single_fields():
     """Full-size fields (2400x1200 in 2D, 256x256x64 in 3D) in one batch: every stage runs once
     over all fields with a work list far larger than the GPU, and every field equals its
