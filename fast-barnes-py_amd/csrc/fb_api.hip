@@ -218,30 +218,34 @@ int pass_geometry(int64_t L, int T, int seg, int lpf, int &lpl, int &lines, int 
     const size_t reserve = 64 * sizeof(float);
     const int lane_lines = lpf / lpl;
     const size_t budget = kLdsBudget / (kBlock / lpf);
+    // LDS: T+1 zeros, then lines of L cells each followed by a gap of rpad floats, v lines
+    // then w lines.  The gap's last T+1 zeros are the next line's left padding (the
+    // reference's zero padding on both sides of a line, shared by neighbouring lines).
+    const size_t lead = (size_t)(T + 1) * sizeof(float);
     auto fit = [&](int rp) {
-        const size_t per_line = (size_t)(L + (int64_t)T + 1 + rp) * 2 * sizeof(float);
-        int64_t n = (int64_t)((budget - reserve) / per_line);
+        const size_t per_line = (size_t)(L + rp) * 2 * sizeof(float);
+        int64_t n = (int64_t)((budget - reserve - lead) / per_line);
         n = std::min<int64_t>(n, lane_lines);
         if (n >= 16) n &= ~(int64_t)3;   // float4 column tiles, at a loss below 1/4
         return (int)std::max<int64_t>(n, 1);
     };
-    // Zeros after each line: the reference's T+1.  A line of several segments may instead
-    // widen them to the overflow of its last segment (seg - L mod seg), so that the
-    // lane whose segment crosses the line end stores all of it and the pass re-zeroes the
-    // padding, instead of predicating every store.  The wider padding is taken when it does
-    // not cost lines per workgroup.
+    // Gap: the reference's T+1 zeros.  A line of several segments may instead widen it by the
+    // overflow of its last segment (seg - L mod seg), so that the lane whose segment crosses
+    // the line end stores all of it (into the widening, clear of the next line's padding) and
+    // the pass re-zeroes what the line reads, instead of predicating every store.  The wider
+    // gap is taken when it does not cost lines per workgroup.
     const int over = (int)((seg - L % seg) % seg);
     rpad = T + 1;
     lines = fit(rpad);
-    if (need > 1 && over > T + 1) {
-        const int wide = fit(over);
+    if (need > 1 && over > 0) {
+        const int wide = fit(T + 1 + over);
         if (wide >= lines) {
-            rpad = over;
+            rpad = T + 1 + over;
             lines = wide;
         }
     }
-    const size_t per_line = (size_t)(L + (int64_t)T + 1 + rpad) * 2 * sizeof(float);
-    lds = round_bytes(lines * per_line + reserve);
+    const size_t per_line = (size_t)(L + rpad) * 2 * sizeof(float);
+    lds = round_bytes(lead + lines * per_line + reserve);
     if (lds > kLdsMax)
         return fail(FB_ERR_UNSUPPORTED, "line of length " + std::to_string(L) + " with half kernel " +
                                             std::to_string(T) + " does not fit the local data share");
@@ -265,7 +269,7 @@ double pass_cost(int64_t L, int T, int seg, int lpf = kBlock, bool *ok = nullptr
     if (ok) *ok = rc == FB_OK;
     if (rc) return 1e300;
     const int over = (int)((seg - L % seg) % seg);
-    const bool full = L > seg && rpad >= over;
+    const bool full = L > seg && rpad - (T + 1) >= over;
     return (7.0 * seg + 2.0 * (2.0 * T + 1.0) + 30.0 + (full ? 0.0 : 2.0 * seg)) / lines * (kBlock / lpf);
 }
 
@@ -672,12 +676,13 @@ int make_field(const fb_field &f, FieldDev &d, int fast)
 // model); the work items are grouped by (kind, segment length, workgroup size), one launch
 // per group.
 int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<PassField> &pf,
-               std::vector<PassGroup> &groups)
+               std::vector<PassGroup> &groups, const std::vector<char> *sel = nullptr)
 {
     pf.assign(P->nfields, PassField{});
     std::map<int, std::vector<PassWork>> by_seg;
     std::map<int, size_t> lds_of;
     for (int f = 0; f < P->nfields; ++f) {
+        if (sel && !(*sel)[f]) continue;   // a pass over the selected fields only (autotuning)
         const FieldDev &F = P->fh[f];
         const int64_t W = F.size[0], H = F.size[1], D = F.size[2];
         ChunkGeom cg;
@@ -1226,20 +1231,12 @@ int fb_plan_backward(fb_plan *P, const double *d_pts, const double *d_val, const
     return rc ? rc : rc2;
 }
 
-// Device time (ms, mean of 5 after a warm-up) of the work items of the fields selected by
-// `sel` in the pass `work`/`pf`/`groups` along axis a, on the plan's own buffers.
-static int time_pass_subset(fb_plan *P, int a, const std::vector<PassWork> &work, const std::vector<PassField> &pf,
-                            const std::vector<PassGroup> &groups, const std::vector<char> &sel, const double *pts,
-                            const double *val, float *out, hipStream_t s, float *ms)
+// Device time (ms, mean of 5 after a warm-up) of the pass `fw`/`pf`/`fg` along axis a (built
+// by build_pass, for all fields or a selection of them), on the plan's own buffers.
+static int time_pass(fb_plan *P, int a, const std::vector<PassWork> &fw, const std::vector<PassField> &pf,
+                     const std::vector<PassGroup> &fg, const double *pts, const double *val, float *out,
+                     hipStream_t s, float *ms)
 {
-    std::vector<PassWork> fw;
-    std::vector<PassGroup> fg;
-    for (const PassGroup &g : groups) {
-        const int first = (int)fw.size();
-        for (int i = g.first; i < g.first + g.count; ++i)
-            if (sel[work[i].field]) fw.push_back(work[i]);
-        if ((int)fw.size() > first) fg.push_back({g.seg, g.lpf, g.kind, first, (int)fw.size() - first, g.lds});
-    }
     *ms = 0.f;
     if (fw.empty()) return FB_OK;
     Scratch sc;
@@ -1319,19 +1316,23 @@ int fb_plan_autotune(fb_plan *P, const double *d_pts, const double *d_val, float
                     if (cl.second[f]) choice_cost(cl.first, P->fh[f].T[a], cand, &ok);
                 if (ok) adm.push_back(cand);
             }
-            // two interleaved rounds over the candidates, each keeping its faster time, so that
-            // clock drift during the sweep favours none of them
+            // Each candidate is timed on a pass over the class's fields alone (built, grouped and
+            // XCD-ordered as a pass of its own: a subset of a pass shared with other classes
+            // would lose the XCD order of its work items), in two interleaved rounds keeping
+            // each candidate's faster time, so that clock drift during the sweep favours none.
+            auto time_choice = [&](const PassChoice &c, const std::vector<char> *sel, float *ms) {
+                P->seg_choice[{a, cl.first}] = c;
+                std::vector<PassWork> work;
+                std::vector<PassField> pf;
+                std::vector<PassGroup> groups;
+                int r = build_pass(P, a, work, pf, groups, sel);
+                return r ? r : time_pass(P, a, work, pf, groups, d_pts, d_val, d_out, s, ms);
+            };
             std::vector<float> tms(adm.size(), 0.f);
             for (int round = 0; round < 2; ++round)
                 for (size_t c = 0; c < adm.size(); ++c) {
-                    P->seg_choice[{a, cl.first}] = adm[c];
-                    std::vector<PassWork> work;
-                    std::vector<PassField> pf;
-                    std::vector<PassGroup> groups;
-                    if ((rc = build_pass(P, a, work, pf, groups))) return rc;
                     float ms = 0.f;
-                    if ((rc = time_pass_subset(P, a, work, pf, groups, cl.second, d_pts, d_val, d_out, s, &ms)))
-                        return rc;
+                    if ((rc = time_choice(adm[c], &cl.second, &ms))) return rc;
                     tms[c] = round == 0 ? ms : std::min(tms[c], ms);
                 }
             if (!adm.empty()) {
@@ -1341,6 +1342,19 @@ int fb_plan_autotune(fb_plan *P, const double *d_pts, const double *d_val, float
                     if (tms[c] < tms[im]) im = c;
                 }
                 best = same(adm[ib], base) && !(tms[im] < 0.98f * tms[ib]) ? adm[ib] : adm[im];
+                // A winner other than the base is confirmed on the whole pass (all classes, two
+                // interleaved rounds): it is kept only if the pass is not slower with it.
+                if (!same(best, adm[ib]) && same(adm[ib], base) && classes.size() > 1) {
+                    float tb = 0.f, tw = 0.f;
+                    for (int round = 0; round < 2; ++round) {
+                        float m0 = 0.f, m1 = 0.f;
+                        if ((rc = time_choice(base, nullptr, &m0)) || (rc = time_choice(best, nullptr, &m1)))
+                            return rc;
+                        tb = round == 0 ? m0 : std::min(tb, m0);
+                        tw = round == 0 ? m1 : std::min(tw, m1);
+                    }
+                    if (!(tw <= tb)) best = base;
+                }
             }
             if (best.seg < 0) P->seg_choice.erase({a, cl.first});
             else P->seg_choice[{a, cl.first}] = best;

@@ -68,7 +68,7 @@ struct PassField {
     int64_t elem_stride;   // offset between consecutive elements of a line
     int32_t T;
     int32_t lpl;           // lanes per line (power of two)
-    int32_t rpad;          // zeros after each line in LDS (>= T+1)
+    int32_t rpad;          // gap after each line's cells in LDS (>= T+1; its last T+1 zeros are the next line's left padding)
     int32_t lines;         // line capacity of a workgroup (lines * lpl <= 256)
     double alpha;
     double pscale;         // (1/(K+2*alpha))^num_iter: the pass's total normalisation

@@ -1165,9 +1165,13 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     const int T = P.T;
     const int lpl = P.lpl;
     const int lines_cap = P.lines;
+    // LDS: T+1 zeros, then per line its P.len cells and a gap of P.rpad floats: line l's
+    // overflow area (full stores, below) followed by zeros, whose last T+1 are line l+1's
+    // left padding.  The w lines follow the v lines, their first left padding the end of the
+    // last v line's gap.
     const int pad = T + 1;
     const int rpad = P.rpad;
-    const int lstride = (int)P.len + pad + rpad;
+    const int lstride = (int)P.len + rpad;
     const int nl = wk.nlines;
     const int64_t es = P.elem_stride;
     float *sv = reinterpret_cast<float *>(smem);
@@ -1184,38 +1188,43 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     const int64_t line_in_field = INJECT ? wk.base / F.size[0] + my_line : 0;
 
     if (INJECT) {
-        // each line's lanes zero the line's own storage (padding included); lines past nl
-        // and the reserve after the w lines feed only discarded steps
+        // each line's lanes zero the line's left padding, cells and gap (the left padding is
+        // the previous line's gap end, which that line also zeroes and never writes again:
+        // zeros over zeros); lines past nl and the reserve after the w lines feed only
+        // discarded steps
         if (line_ok) {
             float *z = mine + (size_t)my_line * lstride;
+            const int zlen = pad + lstride;
             const int head = (int)((16 - ((uintptr_t)z & 15u)) & 15u) >> 2;   // floats before a 16-byte boundary
-            const int nq = (lstride - head) >> 2;
+            const int nq = (zlen - head) >> 2;
             float4 *z4 = reinterpret_cast<float4 *>(z + head);
             for (int i = sid; i < head; i += lpl) z[i] = 0.f;
             for (int i = sid; i < nq; i += lpl) z4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            for (int i = head + 4 * nq + sid; i < lstride; i += lpl) z[i] = 0.f;
+            for (int i = head + 4 * nq + sid; i < zlen; i += lpl) z[i] = 0.f;
         }
         line_sync(lpl);
     } else {
-        const int npad = pad + rpad;
-        if (npad <= 64) {
-            // wave w zeroes the pads of lines w, w + waves, ... (lane = pad position)
+        // zero what follows each line's cells (the rest of a chunk's cells and the gap) and the
+        // left padding of the v lines
+        const int nz = lstride - L;
+        if (nz <= 64) {
+            // wave w zeroes the gaps of lines w, w + waves, ... (lane = position in the gap)
             const int lane = tid & 63;
-            if (lane < npad) {
-                const int o0 = lane < pad ? lane : L + lane;
+            if (lane < nz) {
                 for (int l = tid >> 6; l < lines_cap; l += nthr >> 6) {
-                    sv[l * lstride + o0] = 0.f;
-                    sw[l * lstride + o0] = 0.f;
+                    sv[pad + l * lstride + L + lane] = 0.f;
+                    sw[pad + l * lstride + L + lane] = 0.f;
                 }
             }
         } else {
-            for (int i = tid; i < lines_cap * npad; i += nthr) {
-                const int l = i / npad, r = i - l * npad;
-                const int o = l * lstride + (r < pad ? r : L + r);
+            for (int i = tid; i < lines_cap * nz; i += nthr) {
+                const int l = i / nz, r = i - l * nz;
+                const int o = pad + l * lstride + L + r;
                 sv[o] = 0.f;
                 sw[o] = 0.f;
             }
         }
+        for (int i = tid; i < pad; i += nthr) sv[i] = 0.f;
         if (adj_g)
             adjoint_load(out + F.out_off + wk.base + lo, wsave + F.out_off + wk.base + lo,
                          adj_g + F.out_off + wk.base + lo, off, P.line_stride, nl, L, sv + pad, sw + pad, lstride);
@@ -1246,15 +1255,16 @@ __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     float *ln = mine + (size_t)(line_ok ? my_line : 0) * lstride + pad;
     const double alpha = P.alpha;
     const int niter = F.num_iter;
-    // whole-segment stores (when the host widened the right padding to the overflow of the
-    // last segment): the segment crossing the line end writes `over` floats into the
-    // padding; the line's lanes then re-zero those of them the next iteration reads (within
-    // the first T+1), after the stores: by the wave's program order when the line lies in one
-    // wave, else after a workgroup barrier
+    // whole-segment stores (when the host widened the gap by the overflow of the last
+    // segment): the segment crossing the line end writes `over` floats into the gap's
+    // overflow area, short of the next line's left padding; the line's lanes then re-zero
+    // those of them the next iteration reads (within the first T+1), after the stores: by the
+    // wave's program order when the line lies in one wave, else after a workgroup barrier
     const int over = (kSegArr - L % kSegArr) % kSegArr;
-    const bool full = L > kSegArr && rpad >= over;
+    const bool full = L > kSegArr && rpad - pad >= over;   // the overflow stays clear of the next left padding
     const int nclr = full ? min(pad, over) : 0;
-    // An iteration reads and writes only its own line (reads past it feed discarded steps),
+    // An iteration reads and writes only its own line and its left padding, which no lane
+    // writes after the zeroing (reads past the line feed discarded steps),
     // so the lines synchronise on their own: by wave order when a line's lanes lie in one
     // wave (lpl <= 64, a power of two; each field's half of the workgroup is whole waves),
     // else by workgroup barriers.  The tile IO after the loop spans all lines.
