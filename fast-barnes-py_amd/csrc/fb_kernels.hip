@@ -296,40 +296,84 @@ __global__ __launch_bounds__(kBlock) void k_point_keys_hist(const FieldDev *__re
     __syncthreads();
     int rr[kKeysMaxPpt];
     uint32_t lr[kKeysMaxPpt];
-    // a thread's points ascend: its field is found once and then advanced
+    // a thread's points ascend: its field is found once and then advanced, and the field's
+    // geometry stays in registers until its points pass into the next field
     int f = p0 + threadIdx.x < p1 ? find_seg(pt_prefix, nfields, p0 + threadIdx.x) : 0;
     int64_t fnext = f + 1 < nfields ? pt_prefix[f + 1] : INT64_MAX;
+    const int dim = fields[0].dim;   // the same for every field of a plan
+    double gx0[3], gst[3];
+    int64_t gsz[3], gcell = 0, gxl = 0;
+    auto geometry = [&](int ff) {
+        const FieldDev &F = fields[ff];
 #pragma unroll
-    for (int k = 0; k < kKeysMaxPpt; ++k) {
-        rr[k] = -1;
-        lr[k] = 0u;
-        const int64_t i = p0 + (int64_t)k * kBlock + threadIdx.x;
-        if (k >= ppt || i >= p1) continue;
-        while (i >= fnext) {
-            ++f;
-            fnext = f + 1 < nfields ? pt_prefix[f + 1] : INT64_MAX;
+        for (int m = 0; m < 3; ++m) {
+            gx0[m] = F.x0[m];
+            gst[m] = F.step[m];
+            gsz[m] = F.size[m];
         }
-        const FieldDev &F = fields[f];
-        PointPos pp;
-        const bool inside = point_pos(F, pts + i * F.dim, pp);
-        if (!slot) vals[i] = (uint32_t)i;
-        if (!inside) {
-            keys[i] = kInvalidKey;
-            if (rowg) rowg[i] = kInvalidKey;
-            continue;
+        gcell = F.cell_off;
+        gxl = F.xline_off;
+    };
+    geometry(f);
+    // kKeysGrp points at a time: their coordinates are loaded first, all in flight together
+    constexpr int kKeysGrp = 4;
+#pragma unroll
+    for (int k0 = 0; k0 < kKeysMaxPpt; k0 += kKeysGrp) {
+        double c[kKeysGrp][3];
+#pragma unroll
+        for (int u = 0; u < kKeysGrp; ++u) {
+            const int64_t i = p0 + (int64_t)(k0 + u) * kBlock + threadIdx.x;
+            const bool ok = k0 + u < ppt && i < p1;
+#pragma unroll
+            for (int m = 0; m < 3; ++m) c[u][m] = ok && m < dim ? pts[i * dim + m] : 0.0;
         }
-        const int64_t row = pp.i[2] * F.size[1] + pp.i[1];
-        keys[i] = (uint32_t)(F.cell_off + row * F.size[0] + pp.i[0]);
-        if (rec) rec[i] = PointRec{{pp.w[0], pp.w[1], pp.w[2]}, (int32_t)pp.i[0], 0};
-        const int64_t grow = F.xline_off + row;
-        if (span >= 0) {
-            rr[k] = (int)(grow - rbase);
-            lr[k] = atomicAdd(&hist[rr[k]], 1u);
-        } else {
-            const uint32_t sl = atomicAdd(&row_count[grow], 1u);
-            if (slot) {
-                slot[i] = sl;
-                rowg[i] = (uint32_t)grow;
+#pragma unroll
+        for (int u = 0; u < kKeysGrp; ++u) {
+            const int k = k0 + u;
+            rr[k] = -1;
+            lr[k] = 0u;
+            const int64_t i = p0 + (int64_t)k * kBlock + threadIdx.x;
+            if (k >= ppt || i >= p1) continue;
+            if (i >= fnext) {
+                while (i >= fnext) {
+                    ++f;
+                    fnext = f + 1 < nfields ? pt_prefix[f + 1] : INT64_MAX;
+                }
+                geometry(f);
+            }
+            // point_pos on the cached geometry
+            bool inside = true;
+            int64_t ci[3] = {0, 0, 0};
+            double w[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                if (m < dim) {
+                    const double cc = (c[u][m] - gx0[m]) / gst[m];
+                    if (!(cc >= 0.0) || cc >= (double)(gsz[m] - 1)) inside = false;
+                    const int64_t q = inside ? (int64_t)cc : 0;
+                    ci[m] = q;
+                    w[m] = cc - (double)q;
+                }
+            }
+            if (!slot) vals[i] = (uint32_t)i;
+            if (!inside) {
+                keys[i] = kInvalidKey;
+                if (rowg) rowg[i] = kInvalidKey;
+                continue;
+            }
+            const int64_t row = ci[2] * gsz[1] + ci[1];
+            keys[i] = (uint32_t)(gcell + row * gsz[0] + ci[0]);
+            if (rec) rec[i] = PointRec{{w[0], w[1], w[2]}, (int32_t)ci[0], 0};
+            const int64_t grow = gxl + row;
+            if (span >= 0) {
+                rr[k] = (int)(grow - rbase);
+                lr[k] = atomicAdd(&hist[rr[k]], 1u);
+            } else {
+                const uint32_t sl = atomicAdd(&row_count[grow], 1u);
+                if (slot) {
+                    slot[i] = sl;
+                    rowg[i] = (uint32_t)grow;
+                }
             }
         }
     }
@@ -1796,12 +1840,12 @@ int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int 
 {
     if (npoints <= 0) return FB_OK;
     // points per thread: enough for each workgroup to see many points per row, while keeping
-    // at least ~512 workgroups; small point sets take one atomic per point (workgroups of
-    // k_point_keys_hist whose rows exceed the LDS counters do too, for several points)
+    // at least ~512 workgroups; small point sets, and fields with more rows than the LDS
+    // counters (a workgroup of k_point_keys_hist over them would take one atomic per point, for
+    // several points per thread), take one atomic per point and one point per thread
     const int64_t q = npoints / (512 * (int64_t)kBlock);
     const int64_t ppt = q < kKeysMaxPpt ? q : kKeysMaxPpt;
-    (void)max_rows;
-    if (ppt >= 4 && d_row_count) {
+    if (ppt >= 4 && d_row_count && max_rows <= kKeysBins) {
         const int64_t per = ppt * kBlock;
         hipLaunchKernelGGL(k_point_keys_hist, dim3((unsigned)((npoints + per - 1) / per)), dim3(kBlock), 0, s,
                            d_fields, d_pt_prefix, nfields, npoints, (int)ppt, d_pts, d_keys, d_vals, d_row_count,
