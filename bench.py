@@ -10,7 +10,9 @@ before the timed region; each step replays the batch's captured hipGraph.
 
 Printed JSON line: value = grid points per second of the whole batch; roofline = the
 dominant kernel's algorithmic bytes over its average duration (HIP events on its
-stream), with frac_moved at the bytes this build's kernels move (w kept in float32);
+stream), with frac_moved at the bytes this build's kernels move (w kept in float32) and
+frac_of_copy against this GPU's own copy bandwidth (a 1 GiB device-to-device copy, read +
+write bytes, timed in the same run; copy_gbs);
 cpu_baseline = the CPU oracle (FP64 C restatement of the reference path, -O3, OpenMP
 over lines) on a bounded sample of the same fields, on all host cores and on one.  'workloads' adds the other
 workloads, each with its throughput, pipeline roofline fraction and error against the
@@ -141,6 +143,20 @@ def _time_steps(fn, stream, steps, warmup):
     e1.record(stream)
     e1.synchronize()
     return e0.elapsed_ms(e1) / steps
+
+
+def _copy_bandwidth(stream, mib=1024, reps=10):
+    """Achievable HBM bandwidth of this GPU: a device-to-device copy of `mib` MiB (the runtime's
+    copy kernel), timed with HIP events on `stream`; bytes read + bytes written per second."""
+    from fastbarnes import hip
+    nb = mib << 20
+    a = hip.DeviceArray((nb // 4,), np.float32)
+    b = hip.DeviceArray((nb // 4,), np.float32)
+    cp = hip._load().hipMemcpyAsync
+    ms = _time_steps(lambda: hip.check(cp(b.handle, a.handle, nb, hip.D2D, stream.ptr)), stream, reps, 2)
+    a.free()
+    b.free()
+    return 2.0 * nb / (ms * 1e-3) / 1e9
 
 
 def _errors(out, ref):
@@ -478,17 +494,21 @@ def main():
         if t and t.get('fields') == args.fields:
             traffic = t.get('bytes_per_launch')
             traffic_src = tj.get('source')
+    # the GPU's achievable bandwidth beside the published peak
+    copy_gbs = _copy_bandwidth(stream)
     roof = dict(bound='hbm', kernel=kern, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=traffic_src,
                 alg_bytes_per_launch=alg_bytes, avg_launch_ms=round(stage_ms[kern], 5),
                 moved_bytes_per_launch=moved_bytes,
-                frac_moved=round(moved_bytes / (stage_ms[kern] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+                frac_moved=round(moved_bytes / (stage_ms[kern] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                copy_gbs=round(copy_gbs, 1), frac_of_copy=round(achieved / copy_gbs, 4))
     pipe_bytes = 28.0 * npts + 24.0 * nsamples
     pipe_moved = 20.0 * npts + 24.0 * nsamples
     pipeline = dict(alg_bytes_per_step=pipe_bytes, achieved=round(pipe_bytes / (ms_step * 1e-3) / 1e9, 1),
                     frac=round(pipe_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                     moved_bytes_per_step=pipe_moved,
                     frac_moved=round(pipe_moved / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    frac_of_copy=round(pipe_bytes / (ms_step * 1e-3) / 1e9 / copy_gbs, 4),
                     stage_ms={k: round(v, 5) for k, v in stage_ms.items()})
 
     extras = {}
