@@ -120,6 +120,45 @@ def test_batch_gradients_equal_single_plans():
         np.testing.assert_array_equal(gpts[p0:p1], gp1)
 
 
+@pytest.mark.parametrize('n', [5000, 30000])
+def test_backward_after_another_forward(n):
+    """The backward depends only on its own arguments, not on the plan's last run: after a
+    forward of other points on the same plan (its sorted lists, value offsets and planes then
+    belong to those) it gives the same bits as right after its own forward, for plans that
+    sort with the field sort (n = 5000) and with row buckets (n = 30000)."""
+    from fastbarnes import hip, workloads
+    from fastbarnes.batched import BarnesBatch
+    rng = np.random.default_rng(n)
+    c = workloads.config5(seed=71, n=n, size=(600, 300))
+    c['step'] = 75.0 / 600
+    other = c['pts'] + rng.normal(0.0, 3.0, c['pts'].shape)   # other positions, some outside
+    batch = BarnesBatch([_desc(c)], precision='fp32c')
+
+    def gfun(out):
+        return np.where(np.isnan(out), 0.0, np.sin(np.arange(out.size) * 7e-4))
+
+    out, gval, gpts = _run(batch, c['pts'], c['val'], gfun)
+    d_pts, d_val = hip.DeviceArray.from_numpy(c['pts']), hip.DeviceArray.from_numpy(c['val'])
+    d_oth = hip.DeviceArray.from_numpy(other)
+    d_out = hip.DeviceArray((batch.total_out,), np.float32)
+    d_ws = hip.DeviceArray((batch.total_out,), np.float32)
+    d_o2 = hip.DeviceArray((batch.total_out,), np.float32)
+    d_w2 = hip.DeviceArray((batch.total_out,), np.float32)
+    s = hip.Stream()
+    batch.forward(d_pts.ptr, d_val.ptr, d_out.ptr, d_ws.ptr, s.ptr)
+    batch.forward(d_oth.ptr, d_val.ptr, d_o2.ptr, d_w2.ptr, s.ptr)   # the plan's last sort: other points
+    s.synchronize()
+    np.testing.assert_array_equal(d_out.numpy().view(np.uint32), out.view(np.uint32))
+    d_g = hip.DeviceArray.from_numpy(gfun(out).astype(np.float32))
+    d_gval = hip.DeviceArray((n,), np.float64)
+    d_gpts = hip.DeviceArray(c['pts'].shape, np.float64)
+    batch.backward(d_pts.ptr, d_val.ptr, d_out.ptr, d_ws.ptr, d_g.ptr, d_gval.ptr, d_gpts.ptr, s.ptr)
+    s.synchronize()
+    np.testing.assert_array_equal(d_gval.numpy(), gval)
+    np.testing.assert_array_equal(d_gpts.numpy(), gpts)
+    assert np.count_nonzero(gval) > n // 2
+
+
 def test_torch_autograd_function():
     torch = pytest.importorskip('torch')
     from fastbarnes import workloads
