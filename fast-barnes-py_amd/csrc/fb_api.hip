@@ -253,7 +253,7 @@ int pass_geometry(int64_t L, int T, int seg, int lpf, int &lpl, int &lines, int 
 }
 
 // Segment lengths of the fast-pass instantiations (FB_FAST_SEGS of fb_kernels.hip).
-constexpr int kSegs[] = {25, 29, 33, 37, 39, 41, 45, 49};
+constexpr int kSegs[] = {19, 21, 23, 25, 29, 33, 37, 39, 41, 45, 49};
 
 // Cost model of a fast pass per line: every lane of a workgroup issues the same
 // instructions, about 7 per element of its segment (slide), 2 per window term of the

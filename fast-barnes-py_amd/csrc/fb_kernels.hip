@@ -664,7 +664,7 @@ __global__ __launch_bounds__(kBlock) void k_mask(const FieldDev *__restrict__ fi
 
 // Segment lengths (elements per lane) the fast pass is instantiated for; odd, so the LDS
 // lane stride is conflict-free.  The host picks one per pass and line length.
-#define FB_FAST_SEGS(X) X(25) X(29) X(33) X(37) X(39) X(41) X(45) X(49)
+#define FB_FAST_SEGS(X) X(19) X(21) X(23) X(25) X(29) X(33) X(37) X(39) X(41) X(45) X(49)
 constexpr int kFastBlock = 512;   // threads of a fast-pass workgroup: 256 per field
 constexpr int kBlk = 4;           // add-stream block (LDS loads in flight)
 
