@@ -259,24 +259,26 @@ __global__ __launch_bounds__(kBlock) void k_point_keys_offsets(const FieldDev *_
 }
 
 // k_point_keys for large point sets: each workgroup takes ppt*kBlock consecutive points and,
-// when the rows of the fields they belong to number at most kKeysBins, counts them per row in
+// when the rows of the fields they belong to number at most nbins, counts them per row in
 // LDS first; one global atomic per row the workgroup touches then reserves the row's slots for
 // all of them (instead of one contended atomic per point), and each point's slot is that base
 // plus its rank among the workgroup's points of the row.  The slots come in another order than
 // with one atomic per point; the row sort orders every row by (base cell, point index), so the
 // sorted list is the same.  Workgroups whose rows do not fit count with one atomic per point.
 constexpr int kKeysMaxPpt = 16;     // points per thread, at most
-constexpr int kKeysBins = 8192;     // LDS row counters of a workgroup (32 KB)
+constexpr int kKeysBins = 8192;     // LDS row counters of a workgroup (32 KB) ...
+constexpr int kKeysBinsMax = 16384; // ... or, for plans with fields of up to this many rows, 64 KB
 
 __global__ __launch_bounds__(kBlock) void k_point_keys_hist(const FieldDev *__restrict__ fields,
                                                             const int64_t *__restrict__ pt_prefix, int nfields,
-                                                            int64_t npoints, int ppt, const double *__restrict__ pts,
+                                                            int64_t npoints, int ppt, int nbins,
+                                                            const double *__restrict__ pts,
                                                             uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                                             uint32_t *__restrict__ row_count,
                                                             uint32_t *__restrict__ slot, uint32_t *__restrict__ rowg,
                                                             PointRec *__restrict__ rec)
 {
-    __shared__ uint32_t hist[kKeysBins];
+    extern __shared__ uint32_t hist[];   // nbins row counters (dynamic LDS)
     __shared__ int64_t s_rbase;
     __shared__ int s_span;
     const int64_t p0 = (int64_t)blockIdx.x * ppt * kBlock;
@@ -287,7 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_point_keys_hist(const FieldDev *__re
         const int64_t r0 = A.xline_off;
         const int64_t span = B.xline_off + B.size[1] * B.size[2] - r0;
         s_rbase = r0;
-        s_span = span <= kKeysBins ? (int)span : -1;
+        s_span = span <= nbins ? (int)span : -1;
     }
     __syncthreads();
     const int64_t rbase = s_rbase;
@@ -1845,11 +1847,14 @@ int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int 
     // several points per thread), take one atomic per point and one point per thread
     const int64_t q = npoints / (512 * (int64_t)kBlock);
     const int64_t ppt = q < kKeysMaxPpt ? q : kKeysMaxPpt;
-    if (ppt >= 4 && d_row_count && max_rows <= kKeysBins) {
+    if (ppt >= 4 && d_row_count && max_rows <= kKeysBinsMax) {
+        // 64 KB of counters only where the fields need them (they halve the workgroups per CU;
+        // fast_pass_init raised the kernel's dynamic-LDS limit)
+        const int nbins = max_rows <= kKeysBins ? kKeysBins : kKeysBinsMax;
         const int64_t per = ppt * kBlock;
-        hipLaunchKernelGGL(k_point_keys_hist, dim3((unsigned)((npoints + per - 1) / per)), dim3(kBlock), 0, s,
-                           d_fields, d_pt_prefix, nfields, npoints, (int)ppt, d_pts, d_keys, d_vals, d_row_count,
-                           d_slot, d_rowg, d_rec);
+        hipLaunchKernelGGL(k_point_keys_hist, dim3((unsigned)((npoints + per - 1) / per)), dim3(kBlock),
+                           (size_t)nbins * 4, s, d_fields, d_pt_prefix, nfields, npoints, (int)ppt, nbins, d_pts,
+                           d_keys, d_vals, d_row_count, d_slot, d_rowg, d_rec);
     } else {
         hipLaunchKernelGGL(k_point_keys, dim3(nblocks(npoints)), dim3(kBlock), 0, s, d_fields, d_pt_prefix, nfields,
                            npoints, d_pts, d_keys, d_vals, d_row_count, d_slot, d_rowg, d_rec);
@@ -1903,8 +1908,8 @@ int launch_mask(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfie
 
 int fast_pass_max_lds() { return 160 * 1024; }
 
-// Raises the dynamic-LDS limit of the fast passes once per process (before any
-// stream capture, where attribute calls are not allowed).
+// Raises the dynamic-LDS limit of the fast passes and of k_point_keys_hist once per process
+// (before any stream capture, where attribute calls are not allowed).
 int fast_pass_init()
 {
     static std::once_flag once;
@@ -1922,6 +1927,9 @@ int fast_pass_init()
             err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, fast_pass_max_lds());
             if (err != hipSuccess) break;
         }
+        if (err == hipSuccess)   // the point keys' row counters
+            err = hipFuncSetAttribute((const void *)k_point_keys_hist, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      kKeysBinsMax * (int)sizeof(uint32_t));
     });
     if (err != hipSuccess) {
         (void)hipGetLastError();
