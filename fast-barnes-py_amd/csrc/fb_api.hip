@@ -780,13 +780,13 @@ int build_passes(fb_plan *P)
 }
 
 // One fast pass along `axis` over the whole batch: a launch per segment-length group.
-int run_pass(fb_plan *P, int axis, bool inject, const uint32_t *vals, const double *pts, const double *val,
+int run_pass(fb_plan *P, int axis, bool inject, const uint32_t *vals, const double *val,
              float *out, float *wsave, int final_pass, hipStream_t s, const float *adj_g = nullptr)
 {
     for (const PassGroup &g : P->groups[axis]) {
         int rc = launch_fast_pass(g.kind, inject, g.seg, P->d_fields, P->d_offsets, P->d_work[axis] + g.first, g.count,
                                   P->d_pass[axis], 2 * g.lpf, g.lds, P->d_rec, vals, P->d_line_start,
-                                  P->d_line_count, pts, val, P->d_v, P->d_w, out, wsave, final_pass, adj_g, s);
+                                  P->d_line_count, val, P->d_v, P->d_w, out, wsave, final_pass, adj_g, s);
         if (rc) return rc;
     }
     return FB_OK;
@@ -986,7 +986,7 @@ int plan_run(fb_plan *P, const double *pts, const double *val, float *out, float
         static const char *pn[3] = {"pass_x", "pass_y", "pass_z"};
         for (int a = 0; a < P->dim; ++a) {
             const int fin = a == P->dim - 1;
-            if ((rc = run_pass(P, a, a == 0, vs, pts, val, out, fin ? wsave : nullptr, fin, s))) return rc;
+            if ((rc = run_pass(P, a, a == 0, vs, val, out, fin ? wsave : nullptr, fin, s))) return rc;
             stage_mark(P, s, pn[a]);
         }
     }
@@ -1222,7 +1222,7 @@ int fb_plan_backward(fb_plan *P, const double *d_pts, const double *d_val, const
     // the same scaled line passes; the first one forms a = g/W and b = -a(f - offset) in its tile
     // load, so a and b never round-trip through HBM
     for (int a = 0; a < P->dim && !rc; ++a)
-        rc = run_pass(P, a, false, nullptr, nullptr, nullptr, a == 0 ? const_cast<float *>(d_out) : nullptr,
+        rc = run_pass(P, a, false, nullptr, nullptr, a == 0 ? const_cast<float *>(d_out) : nullptr,
                       a == 0 ? const_cast<float *>(d_wsave) : nullptr, 0, s, a == 0 ? d_gout : nullptr);
     if (!rc)
         rc = launch_adj_gather(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints, d_pts, d_val, P->d_offsets,
@@ -1234,7 +1234,7 @@ int fb_plan_backward(fb_plan *P, const double *d_pts, const double *d_val, const
 // Device time (ms, mean of 5 after a warm-up) of the pass `fw`/`pf`/`fg` along axis a (built
 // by build_pass, for all fields or a selection of them), on the plan's own buffers.
 static int time_pass(fb_plan *P, int a, const std::vector<PassWork> &fw, const std::vector<PassField> &pf,
-                     const std::vector<PassGroup> &fg, const double *pts, const double *val, float *out,
+                     const std::vector<PassGroup> &fg, const double *val, float *out,
                      hipStream_t s, float *ms)
 {
     *ms = 0.f;
@@ -1256,7 +1256,7 @@ static int time_pass(fb_plan *P, int a, const std::vector<PassWork> &fw, const s
         if (rep == 1) (void)hipEventRecord(e0, s);
         for (const PassGroup &g : fg) {
             rc = launch_fast_pass(g.kind, a == 0, g.seg, P->d_fields, P->d_offsets, dw + g.first, g.count, dp, 2 * g.lpf,
-                                  g.lds, P->d_rec, P->sorted_vals, P->d_line_start, P->d_line_count, pts, val, P->d_v,
+                                  g.lds, P->d_rec, P->sorted_vals, P->d_line_start, P->d_line_count, val, P->d_v,
                                   P->d_w, out, nullptr, fin, nullptr, s);
             if (rc) break;
         }
@@ -1326,7 +1326,7 @@ int fb_plan_autotune(fb_plan *P, const double *d_pts, const double *d_val, float
                 std::vector<PassField> pf;
                 std::vector<PassGroup> groups;
                 int r = build_pass(P, a, work, pf, groups, sel);
-                return r ? r : time_pass(P, a, work, pf, groups, d_pts, d_val, d_out, s, ms);
+                return r ? r : time_pass(P, a, work, pf, groups, d_val, d_out, s, ms);
             };
             std::vector<float> tms(adm.size(), 0.f);
             for (int round = 0; round < 2; ++round)
@@ -1905,7 +1905,7 @@ int fb_accumulate_lines(int tail, int precision, int64_t nlines, int64_t len, in
         if ((rc = launch_zero(d_w, cells * 4, s))) return rc;
         if ((rc = launch_fast_pass(kPassSlide, false, seg, d_f, d_off, d_work, (int)work.size(), d_pf, 2 * kBlock, lds,
                                    nullptr, nullptr,
-                                   nullptr, nullptr, nullptr, nullptr, d_v, d_w, nullptr, nullptr, 0, nullptr, s)))
+                                   nullptr, nullptr, nullptr, d_v, d_w, nullptr, nullptr, 0, nullptr, s)))
             return rc;
         if ((rc = launch_f32_to_f64(d_v, d_io, cells, 1.0, s))) return rc;
     }
@@ -2000,7 +2000,7 @@ int fb_convolve_grid(int tail, int precision, int dim, const int64_t *size, cons
             FB_HIP_CHECK(hipMemcpy(d_pf, pf.data(), sizeof(PassField), hipMemcpyHostToDevice));
             for (const PassGroup &g : groups)
                 if ((rc = launch_fast_pass(g.kind, false, g.seg, d_f, d_off, d_work + g.first, g.count, d_pf, 2 * g.lpf,
-                                           g.lds, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, d_v, d_w,
+                                           g.lds, nullptr, nullptr, nullptr, nullptr, nullptr, d_v, d_w,
                                            nullptr, nullptr, 0, nullptr, s)))
                     return rc;
             FB_HIP_CHECK(hipStreamSynchronize(s));   // host vectors above go out of scope

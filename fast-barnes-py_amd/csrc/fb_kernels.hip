@@ -1181,7 +1181,7 @@ template <bool INJECT, int SEG>
 __global__ __launch_bounds__(kFastBlock, 1) void k_fast_pass(
     const FieldDev *__restrict__ fields, const double *__restrict__ offsets, const PassWork *__restrict__ work,
     const PassField *__restrict__ passes, const PointRec *__restrict__ rec, const uint32_t *__restrict__ vals,
-    const uint32_t *__restrict__ line_start, const uint32_t *__restrict__ line_count, const double *__restrict__ pts,
+    const uint32_t *__restrict__ line_start, const uint32_t *__restrict__ line_count,
     const double *__restrict__ val, float *__restrict__ gv, float *__restrict__ gw, float *__restrict__ out,
     float *__restrict__ wsave, int final_pass, const float *__restrict__ adj_g)
 {
@@ -1506,7 +1506,7 @@ template <bool INJECT, int SEG>
 __global__ __launch_bounds__(kFastBlock, 1) void k_prefix_pass(
     const FieldDev *__restrict__ fields, const double *__restrict__ offsets, const PassWork *__restrict__ work,
     const PassField *__restrict__ passes, const PointRec *__restrict__ rec, const uint32_t *__restrict__ vals,
-    const uint32_t *__restrict__ line_start, const uint32_t *__restrict__ line_count, const double *__restrict__ pts,
+    const uint32_t *__restrict__ line_start, const uint32_t *__restrict__ line_count,
     const double *__restrict__ val, float *__restrict__ gv, float *__restrict__ gw, float *__restrict__ out,
     float *__restrict__ wsave, int final_pass, const float *__restrict__ adj_g)
 {
@@ -1964,7 +1964,7 @@ bool fast_pass_seg_supported(int kind, int seg)
 int launch_fast_pass(int kind, bool inject, int seg, const FieldDev *d_fields, const double *d_offsets,
                      const PassWork *d_work, int nwork, const PassField *d_pass, int threads, size_t lds_bytes,
                      const PointRec *d_rec, const uint32_t *d_vals, const uint32_t *d_line_start,
-                     const uint32_t *d_line_count, const double *d_pts, const double *d_val, float *d_v, float *d_w,
+                     const uint32_t *d_line_count, const double *d_val, float *d_v, float *d_w,
                      float *d_out, float *d_wsave, int final_pass, const float *d_adj_g, hipStream_t s)
 {
     if (nwork <= 0) return FB_OK;
@@ -1980,11 +1980,11 @@ int launch_fast_pass(int kind, bool inject, int seg, const FieldDev *d_fields, c
     case S:                                                                                                        \
         if (inject)                                                                                                \
             hipLaunchKernelGGL((K<true, S>), dim3(nwork), dim3(threads), lds_bytes, s, d_fields, d_offsets, d_work, \
-                               d_pass, d_rec, d_vals, d_line_start, d_line_count, d_pts, d_val, d_v, d_w, d_out,    \
+                               d_pass, d_rec, d_vals, d_line_start, d_line_count, d_val, d_v, d_w, d_out,           \
                                d_wsave, final_pass, d_adj_g);                                                      \
         else                                                                                                       \
             hipLaunchKernelGGL((K<false, S>), dim3(nwork), dim3(threads), lds_bytes, s, d_fields, d_offsets,        \
-                               d_work, d_pass, d_rec, d_vals, d_line_start, d_line_count, d_pts, d_val, d_v, d_w,   \
+                               d_work, d_pass, d_rec, d_vals, d_line_start, d_line_count, d_val, d_v, d_w,          \
                                d_out, d_wsave, final_pass, d_adj_g);                                               \
         break;
 #define FB_LAUNCH_SLIDE(S) FB_LAUNCH(k_fast_pass, S)

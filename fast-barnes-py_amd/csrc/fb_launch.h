@@ -69,12 +69,14 @@ int launch_mask(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfie
 int fast_pass_max_lds();
 int fast_pass_init();
 bool fast_pass_seg_supported(int kind, int seg);
-// kind: kPassSlide (k_fast_pass) or kPassPrefix (k_prefix_pass).  d_adj_g (first pass of the
-// adjoint): the tile load forms a = g/W, b = -a(f - offset) from d_out = f, d_wsave = W, d_adj_g = g.
+// kind: kPassSlide (k_fast_pass) or kPassPrefix (k_prefix_pass).  Injecting passes: d_vals is the
+// sorted point list (rows' ranges d_line_start/d_line_count), d_rec and d_val each point's
+// injection record and value by point index.  d_adj_g (first pass of the adjoint): the tile
+// load forms a = g/W, b = -a(f - offset) from d_out = f, d_wsave = W, d_adj_g = g.
 int launch_fast_pass(int kind, bool inject, int seg, const FieldDev *d_fields, const double *d_offsets, const PassWork *d_work,
                      int nwork, const PassField *d_pass, int threads, size_t lds_bytes, const PointRec *d_rec,
                      const uint32_t *d_vals, const uint32_t *d_line_start, const uint32_t *d_line_count,
-                     const double *d_pts, const double *d_val, float *d_v, float *d_w, float *d_out,
+                     const double *d_val, float *d_v, float *d_w, float *d_out,
                      float *d_wsave, int final_pass, const float *d_adj_g, hipStream_t stream);
 
 // fb_kernels.hip -- adjoint of the FP32C batch
