@@ -217,7 +217,8 @@ __global__ __launch_bounds__(kLocalSortThreads) void k_field_sort(const FieldDev
 // of the counts, and each row's entries are then sorted by the composite (key, point index).
 
 constexpr int kRowSmall = 16;           // rows of at most this many entries: one thread, in registers
-constexpr int kRowWave = 1024;          // rows of at most this many entries: one wave, in LDS
+constexpr int kRowWave = 1024;          // rows of at most this many entries: one wave, in LDS ...
+constexpr int kRankWave = 256;          // ... by rank up to this many, by a bitonic network above
 constexpr int kRowSortThreads = 256;    // workgroup of the wave kernel (4 rows at a time) and of larger rows
 
 __device__ __forceinline__ unsigned long long row_entry(const uint32_t *keys, const uint32_t *vals, uint32_t i)
@@ -298,8 +299,9 @@ __global__ __launch_bounds__(kThreads) void k_row_sort_small(int64_t nrows, cons
 }
 
 // Waves loop over the rows in list (at most kRowWave entries each): each wave sorts one row at a
-// time by the bitonic network in its own kRowWave-entry slice of LDS.  Within one wave every
-// stage only needs its LDS operations complete (a wavefront fence), never a workgroup barrier.
+// time in its own kRowWave-entry slice of LDS, by rank (rows of at most kRankWave entries) or by
+// the bitonic network.  Within one wave every step only needs its LDS operations complete (a
+// wavefront fence), never a workgroup barrier.
 __global__ __launch_bounds__(kRowSortThreads) void k_row_sort_wave(const uint32_t *__restrict__ row_start,
                                                                    const uint32_t *__restrict__ row_count,
                                                                    uint32_t *__restrict__ skeys,
@@ -317,6 +319,35 @@ __global__ __launch_bounds__(kRowSortThreads) void k_row_sort_wave(const uint32_
         const uint32_t r = list[b];
         const uint32_t n = row_count[r];
         const uint32_t s0 = row_start[r];
+        if (n <= (uint32_t)kRankWave) {
+            // rows of at most kRankWave entries: each entry's slot is its rank, the number of the
+            // row's composites below it (all differ), counted against the row in LDS (every lane
+            // reads the same element at a time: broadcast reads)
+            unsigned long long e[kRankWave / 64];
+            uint32_t rk[kRankWave / 64];
+#pragma unroll
+            for (int u = 0; u < kRankWave / 64; ++u) {
+                const uint32_t i = lane + 64 * u;
+                e[u] = i < n ? row_entry(skeys, svals, s0 + i) : ~0ull;
+                if (i < n) sk[i] = e[u];
+                rk[u] = 0u;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            for (uint32_t j = 0; j < n; ++j) {
+                const unsigned long long x = sk[j];
+#pragma unroll
+                for (int u = 0; u < kRankWave / 64; ++u) rk[u] += x < e[u] ? 1u : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < kRankWave / 64; ++u) {
+                if (lane + 64 * u < (int)n) {
+                    skeys[s0 + rk[u]] = (uint32_t)(e[u] >> 32);
+                    svals[s0 + rk[u]] = (uint32_t)e[u];
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");   // before the next row reuses sk
+            continue;
+        }
         int np2 = 1;
         while ((uint32_t)np2 < n) np2 <<= 1;
         for (int i = lane; i < np2; i += 64) sk[i] = (uint32_t)i < n ? row_entry(skeys, svals, s0 + i) : ~0ull;
