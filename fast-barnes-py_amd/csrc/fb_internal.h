@@ -108,6 +108,16 @@ __host__ __device__ inline int64_t blocked_index(int64_t y, int64_t x, int64_t W
     return (((y >> 3) * (W >> 2) + (x >> 2)) << 5) + ((y & 7) << 2) + (x & 3);
 }
 
+// The adjoint's A and B of such a field, as its y pass leaves them for the gather: (A, B)
+// pairs in blocks of 4 rows x 4 columns, one 128-byte line each, within the footprint of the
+// blocked v and w fields.  The pair of node (y, x) starts at float adj_pair_index(y, x, W) of
+// the v field when y % 8 < 4, else of the w field.  A point's 2x2 corners then touch 1.56
+// lines on average, against 2.81 for A and B at the same blocked index of two fields.
+__host__ __device__ inline int64_t adj_pair_index(int64_t y, int64_t x, int64_t W)
+{
+    return (((y >> 3) * (W >> 2) + (x >> 2)) << 5) + ((y & 3) << 3) + ((x & 3) << 1);
+}
+
 // A point's grid position as the FP32C injection uses it, formed once per point by the
 // point-key kernels (point_pos): its base cell along x and its fractional offsets per axis.
 struct PointRec {

@@ -952,8 +952,10 @@ __device__ __forceinline__ void blocked_load_cols(const float *__restrict__ gv, 
         }
 }
 
-// The inverse of blocked_load_cols: a y pass that is not the last one (the adjoint's) writes
-// its tile back in place, in the blocked layout it was read in.
+// Store of a y pass that is not the last one (the adjoint's, whose v and w lines hold A and
+// B): in place, as (A, B) pairs (adj_pair_index), within the 8-row blocks of the columns
+// x0 .. x0+nl-1 it read, so no other workgroup's nodes are touched.  Thread (q, r) writes the
+// pairs of row r, columns x0+4q .. x0+4q+3: 32 contiguous bytes.
 __device__ __forceinline__ void blocked_store_cols(float *__restrict__ gv, float *__restrict__ gw, int64_t W,
                                                    int64_t x0, int nl, int L, const float *__restrict__ dv,
                                                    const float *__restrict__ dw, int lstride)
@@ -968,9 +970,9 @@ __device__ __forceinline__ void blocked_store_cols(float *__restrict__ gv, float
         const int r = r0 + k * rstep;
         if (r < L) {
             const int e = k * rstep;
-            const int64_t o = blocked_index(r, x0 + 4 * q, W);
-            *reinterpret_cast<float4 *>(gv + o) = make_float4(a[e], a[lstride + e], a[2 * lstride + e], a[3 * lstride + e]);
-            *reinterpret_cast<float4 *>(gw + o) = make_float4(b[e], b[lstride + e], b[2 * lstride + e], b[3 * lstride + e]);
+            float4 *p = reinterpret_cast<float4 *>(((r & 4) ? gw : gv) + adj_pair_index(r, x0 + 4 * q, W));
+            p[0] = make_float4(a[e], b[e], a[lstride + e], b[lstride + e]);
+            p[1] = make_float4(a[2 * lstride + e], b[2 * lstride + e], a[3 * lstride + e], b[3 * lstride + e]);
         }
     }
 }
@@ -1762,7 +1764,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_gather(const FieldDev *__restric
     const int f = find_seg(pt_prefix, nfields, i);
     const FieldDev &F = fields[f];
     const int dim = F.dim;
-    // A and B of a 2D field whose y pass ran on the blocked intermediate are blocked
+    // A and B of a 2D field whose y pass ran on the blocked intermediate are (A, B) pairs
+    // (adj_pair_index), the pairs of rows y % 8 < 4 in A's field and the others in Bb's
     const bool blocked = pass_y && pass_y[f].blocked == kBlockedIn;
     PointPos pp;
     const bool inside = point_pos(F, pts + i * dim, pp);
@@ -1773,9 +1776,18 @@ __global__ __launch_bounds__(kBlock) void k_adj_gather(const FieldDev *__restric
         const int corners = 1 << dim;
         for (int cb = 0; cb < corners; ++cb) {
             const int bx = cb & 1, by = (cb >> 1) & 1, bz = (cb >> 2) & 1;
-            const int64_t cell = F.cell_off + (blocked ? blocked_index(pp.i[1] + by, pp.i[0] + bx, W)
-                                                       : ((pp.i[2] + bz) * H + (pp.i[1] + by)) * W + (pp.i[0] + bx));
-            const double a = (double)A[cell], b = (double)Bb[cell];
+            double a, b;
+            if (blocked) {
+                const int64_t y = pp.i[1] + by;
+                const float2 ab = *reinterpret_cast<const float2 *>((y & 4 ? Bb : A) + F.cell_off +
+                                                                    adj_pair_index(y, pp.i[0] + bx, W));
+                a = (double)ab.x;
+                b = (double)ab.y;
+            } else {
+                const int64_t cell = F.cell_off + ((pp.i[2] + bz) * H + (pp.i[1] + by)) * W + (pp.i[0] + bx);
+                a = (double)A[cell];
+                b = (double)Bb[cell];
+            }
             const double fx = bx ? pp.w[0] : 1.0 - pp.w[0];
             const double fy = dim >= 2 ? (by ? pp.w[1] : 1.0 - pp.w[1]) : 1.0;
             const double fz = dim >= 3 ? (bz ? pp.w[2] : 1.0 - pp.w[2]) : 1.0;
