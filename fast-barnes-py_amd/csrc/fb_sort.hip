@@ -298,6 +298,41 @@ __global__ __launch_bounds__(kThreads) void k_row_sort_small(int64_t nrows, cons
         }
 }
 
+// One wave sorts the row of n <= 64*NU entries at s0 by rank: each entry's slot is the number of
+// the row's composites below it (all differ), counted against the row's copy in LDS `sk`, two
+// composites per read (every lane reads the same pair at a time: broadcast reads; an odd row
+// is padded with ~0, which no composite exceeds).  NU = ceil(n/64) keeps the compares to the
+// registers that hold entries.
+template <int NU>
+__device__ __forceinline__ void rank_row(uint32_t *__restrict__ skeys, uint32_t *__restrict__ svals, uint32_t s0,
+                                         uint32_t n, unsigned long long *__restrict__ sk, int lane)
+{
+    unsigned long long e[NU];
+    uint32_t rk[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+        const uint32_t i = lane + 64 * u;
+        e[u] = i < n ? row_entry(skeys, svals, s0 + i) : ~0ull;
+        if (i < n) sk[i] = e[u];
+        rk[u] = 0u;
+    }
+    if (lane == 0 && (n & 1u)) sk[n] = ~0ull;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const ulonglong2 *sk2 = reinterpret_cast<const ulonglong2 *>(sk);
+    for (uint32_t j = 0; j < (n + 1) >> 1; ++j) {
+        const ulonglong2 x = sk2[j];
+#pragma unroll
+        for (int u = 0; u < NU; ++u) rk[u] += (x.x < e[u] ? 1u : 0u) + (x.y < e[u] ? 1u : 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+        if (lane + 64 * u < (int)n) {
+            skeys[s0 + rk[u]] = (uint32_t)(e[u] >> 32);
+            svals[s0 + rk[u]] = (uint32_t)e[u];
+        }
+    }
+}
+
 // Waves loop over the rows in list (at most kRowWave entries each): each wave sorts one row at a
 // time in its own kRowWave-entry slice of LDS, by rank (rows of at most kRankWave entries) or by
 // the bitonic network.  Within one wave every step only needs its LDS operations complete (a
@@ -309,7 +344,7 @@ __global__ __launch_bounds__(kRowSortThreads) void k_row_sort_wave(const uint32_
                                                                    const uint32_t *__restrict__ list,
                                                                    const uint32_t *__restrict__ count)
 {
-    __shared__ unsigned long long lds[(kRowSortThreads / 64) * kRowWave];
+    __shared__ __align__(16) unsigned long long lds[(kRowSortThreads / 64) * kRowWave];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     unsigned long long *sk = lds + wv * kRowWave;
@@ -320,30 +355,12 @@ __global__ __launch_bounds__(kRowSortThreads) void k_row_sort_wave(const uint32_
         const uint32_t n = row_count[r];
         const uint32_t s0 = row_start[r];
         if (n <= (uint32_t)kRankWave) {
-            // rows of at most kRankWave entries: each entry's slot is its rank, the number of the
-            // row's composites below it (all differ), counted against the row in LDS (every lane
-            // reads the same element at a time: broadcast reads)
-            unsigned long long e[kRankWave / 64];
-            uint32_t rk[kRankWave / 64];
-#pragma unroll
-            for (int u = 0; u < kRankWave / 64; ++u) {
-                const uint32_t i = lane + 64 * u;
-                e[u] = i < n ? row_entry(skeys, svals, s0 + i) : ~0ull;
-                if (i < n) sk[i] = e[u];
-                rk[u] = 0u;
-            }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            for (uint32_t j = 0; j < n; ++j) {
-                const unsigned long long x = sk[j];
-#pragma unroll
-                for (int u = 0; u < kRankWave / 64; ++u) rk[u] += x < e[u] ? 1u : 0u;
-            }
-#pragma unroll
-            for (int u = 0; u < kRankWave / 64; ++u) {
-                if (lane + 64 * u < (int)n) {
-                    skeys[s0 + rk[u]] = (uint32_t)(e[u] >> 32);
-                    svals[s0 + rk[u]] = (uint32_t)e[u];
-                }
+            // rows of at most kRankWave entries (more than kRowSmall): by rank
+            switch ((n + 63) >> 6) {
+            case 1: rank_row<1>(skeys, svals, s0, n, sk, lane); break;
+            case 2: rank_row<2>(skeys, svals, s0, n, sk, lane); break;
+            case 3: rank_row<3>(skeys, svals, s0, n, sk, lane); break;
+            default: rank_row<kRankWave / 64>(skeys, svals, s0, n, sk, lane); break;
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");   // before the next row reuses sk
             continue;

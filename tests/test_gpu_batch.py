@@ -323,7 +323,9 @@ def test_row_buckets_with_crowded_rows():
     row, then each row sorted by (cell, point index) -- by one thread (up to 16 entries), by a
     workgroup in LDS (up to 8192) or by rank (more).  Crowded rows exercise the last two: a
     field with 9000 points on one row and 1500 points on each of 3 rows, besides scattered
-    ones, one with every point on one row, and a tall sparse field.  The FP64 results must equal the oracle's bit for
+    ones, one with every point on one row, and a tall sparse field.  Rows of 17 to 256 entries
+    are ranked by one wave in groups of 64 entries: a field whose rows hold 17 to 256 points,
+    at every group count, odd and even.  The FP64 results must equal the oracle's bit for
     bit (the reference's per-cell point order), and the FP32C path must agree with FP64."""
     from fastbarnes.batched import BarnesBatch
     from oracle import oracle as orc
@@ -340,8 +342,12 @@ def test_row_buckets_with_crowded_rows():
                          np.stack([rng.uniform(-26, 48, 2000), rng.uniform(34, 71, 2000)], axis=1)])
     p1 = p1[rng.permutation(len(p1))]
     p2 = on_rows([33], 12000)
+    counts = [17, 63, 64, 65, 127, 128, 129, 191, 192, 193, 255, 256]
+    p4 = np.concatenate([on_rows([20 + 9 * k], c) for k, c in enumerate(counts)] +
+                        [np.stack([rng.uniform(-26, 48, 9000), rng.uniform(65.25, 71, 9000)], axis=1)])
+    p4 = p4[rng.permutation(len(p4))]
     cases = []
-    for p in (p1, p2):
+    for p in (p1, p2, p4):
         cases.append(dict(pts=p, val=np.sin(p[:, 0] / 3.0) * 10 + p[:, 1], sigma=1.2, x0=x0, step=step,
                           size=(W, H), num_iter=4, max_dist=3.5))
     # a tall sparse field: most rows of the batch hold 0-1 points (the single-thread class)
