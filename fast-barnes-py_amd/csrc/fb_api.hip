@@ -203,6 +203,27 @@ void scale_groups(double g, int n, double total, PassField &p)
     p.slast = ngroups > 1 ? std::pow(1.0 / g, (double)last) : total;
 }
 
+// Most lanes of one wave on one of the 32 LDS banks (4 bytes wide) when a column tile of nl
+// lines with line stride `lstride` passes through LDS in a workgroup of nthr threads: the
+// float4 column path of tile_copy and blocked_load_cols when nl is a multiple of 4 (thread t
+// takes lines 4q .. 4q+3, q = t % (nl/4), at rows t / (nl/4) + k*nthr/(nl/4)), else the scalar
+// path (line t % nl, rows t / nl + ...).  64 lanes on 32 banks: 2 is the least.
+int column_bank_conflicts(int lstride, int nl, int nthr)
+{
+    const bool quad = nl % 4 == 0;
+    const int groups = quad ? nl / 4 : nl;
+    int worst = 0;
+    for (int w = 0; w < nthr / 64; ++w)
+        for (int c = 0; c < (quad ? 4 : 1); ++c) {
+            int bank[32] = {0};
+            for (int t = w * 64; t < w * 64 + 64; ++t) {
+                const int64_t a = (int64_t)(quad ? 4 * (t % groups) + c : t % groups) * lstride + t / groups;
+                worst = std::max(worst, ++bank[a % 32]);
+            }
+        }
+    return worst;
+}
+
 // lanes per line / lines per workgroup of a fast line pass with `seg` elements per lane and
 // workgroups of 2*lpf threads (lpf lanes per field); the LDS budget scales with the
 // workgroup so that every size keeps 16 waves per CU
@@ -242,6 +263,16 @@ int pass_geometry(int64_t L, int T, int seg, int lpf, int &lpl, int &lines, int 
         if (wide >= lines) {
             rpad = T + 1 + over;
             lines = wide;
+        }
+    }
+    // A wider gap changes nothing but the line stride; up to 3 more floats where they lower the
+    // LDS bank conflicts of the column tiles and cost no lines
+    int best = column_bank_conflicts((int)L + rpad, lines, 2 * lpf);
+    for (int extra = 1, base = rpad; extra <= 3 && best > 2; ++extra) {
+        const int c = column_bank_conflicts((int)L + base + extra, lines, 2 * lpf);
+        if (c < best && fit(base + extra) >= lines) {
+            best = c;
+            rpad = base + extra;
         }
     }
     const size_t per_line = (size_t)(L + rpad) * 2 * sizeof(float);
