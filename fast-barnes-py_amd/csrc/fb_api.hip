@@ -203,20 +203,35 @@ void scale_groups(double g, int n, double total, PassField &p)
     p.slast = ngroups > 1 ? std::pow(1.0 / g, (double)last) : total;
 }
 
-// Most lanes of one wave on one of the 32 LDS banks (4 bytes wide) when a column tile of nl
-// lines with line stride `lstride` passes through LDS in a workgroup of nthr threads: the
-// float4 column path of tile_copy and blocked_load_cols when nl is a multiple of 4 (thread t
-// takes lines 4q .. 4q+3, q = t % (nl/4), at rows t / (nl/4) + k*nthr/(nl/4)), else the scalar
-// path (line t % nl, rows t / nl + ...).  64 lanes on 32 banks: 2 is the least.
+// LDS bank multiplicity of a line layout: the most lanes of one half-wave (32 lanes, served
+// together by the 32 four-byte banks; 1 is conflict-free) that meet on one bank.
+//  * slide_bank_conflicts: the slides and their stores, lane sid of line l of a field's half
+//    of the workgroup at l*lstride + sid*seg + j.  Within a line the odd segment length keeps
+//    the lanes apart; lines shorter than 32 lanes share a half-wave, and their lanes fall on
+//    disjoint banks when lstride = seg*lpl*odd (mod 32).
+//  * column_bank_conflicts: the column tiles of a y or z pass (tile_copy, blocked_load_cols),
+//    element (l, r) at l*lstride + r: thread t takes lines 4q .. 4q+3, q = t % (nl/4), at rows
+//    t / (nl/4) + k*nthr/(nl/4) when nl is a multiple of 4 (float4 columns), else line t % nl.
+int slide_bank_conflicts(int lstride, int lpl, int seg, int nl)
+{
+    int worst = 0;
+    for (int t0 = 0; t0 < nl * lpl; t0 += 32) {
+        int bank[32] = {0};
+        for (int t = t0; t < std::min(t0 + 32, nl * lpl); ++t)
+            worst = std::max(worst, ++bank[((int64_t)(t / lpl) * lstride + (int64_t)(t % lpl) * seg) % 32]);
+    }
+    return worst;
+}
+
 int column_bank_conflicts(int lstride, int nl, int nthr)
 {
     const bool quad = nl % 4 == 0;
     const int groups = quad ? nl / 4 : nl;
     int worst = 0;
-    for (int w = 0; w < nthr / 64; ++w)
+    for (int t0 = 0; t0 < nthr; t0 += 32)
         for (int c = 0; c < (quad ? 4 : 1); ++c) {
             int bank[32] = {0};
-            for (int t = w * 64; t < w * 64 + 64; ++t) {
+            for (int t = t0; t < t0 + 32; ++t) {
                 const int64_t a = (int64_t)(quad ? 4 * (t % groups) + c : t % groups) * lstride + t / groups;
                 worst = std::max(worst, ++bank[a % 32]);
             }
@@ -227,7 +242,11 @@ int column_bank_conflicts(int lstride, int nl, int nthr)
 // lanes per line / lines per workgroup of a fast line pass with `seg` elements per lane and
 // workgroups of 2*lpf threads (lpf lanes per field); the LDS budget scales with the
 // workgroup so that every size keeps 16 waves per CU
-int pass_geometry(int64_t L, int T, int seg, int lpf, int &lpl, int &lines, int &rpad, size_t &lds)
+// What pass_geometry's choice of the gap width looks at besides the lines per workgroup: nothing
+// (the cost model), the slides only (x passes: row tiles), the slides then the column tiles
+enum BankSearch { kBanksNone = 0, kBanksRows = 1, kBanksColumns = 2 };
+
+int pass_geometry(int64_t L, int T, int seg, int lpf, int &lpl, int &lines, int &rpad, size_t &lds, int banks)
 {
     const int64_t need = (L + seg - 1) / seg;
     lpl = 1;
@@ -265,14 +284,31 @@ int pass_geometry(int64_t L, int T, int seg, int lpf, int &lpl, int &lines, int 
             lines = wide;
         }
     }
-    // A wider gap changes nothing but the line stride; up to 3 more floats where they lower the
-    // LDS bank conflicts of the column tiles and cost no lines
-    int best = column_bank_conflicts((int)L + rpad, lines, 2 * lpf);
-    for (int extra = 1, base = rpad; extra <= 3 && best > 2; ++extra) {
-        const int c = column_bank_conflicts((int)L + base + extra, lines, 2 * lpf);
-        if (c < best && fit(base + extra) >= lines) {
-            best = c;
-            rpad = base + extra;
+    // A wider gap changes nothing but the line stride: up to 31 more floats, where they cost no
+    // lines, for the fewest LDS bank conflicts of the slides (about 14 LDS accesses per element
+    // and pass in four iterations), then, in passes with column tiles, of those (2)
+    if (banks != kBanksNone) {
+        const int base = rpad;
+        // nor workgroups per CU (the LDS's, up to 32 waves)
+        auto per_cu = [&](int rp) {
+            const size_t b = round_bytes(lead + lines * (size_t)(L + rp) * 2 * sizeof(float) + reserve);
+            return std::min<size_t>(kLdsMax / b, (size_t)(32 / std::max(1, 2 * lpf / 64)));
+        };
+        const size_t wgs = per_cu(base);
+        int sbest = 1 << 30, cbest = 1 << 30;
+        int sc[32];
+        for (int extra = 0; extra < 32; ++extra) {
+            sc[extra] = extra == 0 || (fit(base + extra) >= lines && per_cu(base + extra) >= wgs)
+                            ? slide_bank_conflicts((int)L + base + extra, lpl, seg, lines) : 1 << 30;
+            sbest = std::min(sbest, sc[extra]);
+        }
+        for (int extra = 0; extra < 32 && cbest > 1; ++extra) {
+            if (sc[extra] != sbest) continue;
+            const int c = banks == kBanksColumns ? column_bank_conflicts((int)L + base + extra, lines, 2 * lpf) : 1;
+            if (c < cbest) {
+                cbest = c;
+                rpad = base + extra;
+            }
         }
     }
     const size_t per_line = (size_t)(L + rpad) * 2 * sizeof(float);
@@ -295,7 +331,7 @@ double pass_cost(int64_t L, int T, int seg, int lpf = kBlock, bool *ok = nullptr
     int lpl, lines, rpad;
     size_t lds;
     const std::string keep = last_error();
-    const int rc = pass_geometry(L, T, seg, lpf, lpl, lines, rpad, lds);
+    const int rc = pass_geometry(L, T, seg, lpf, lpl, lines, rpad, lds, kBanksNone);   // the bank search keeps the lines
     set_error(keep);
     if (ok) *ok = rc == FB_OK;
     if (rc) return 1e300;
@@ -629,7 +665,7 @@ bool field_blocked(const fb_plan *P, int f)
     int lpl, lines, rpad;
     size_t lds;
     const std::string keep = last_error();
-    const int rc = pass_geometry(F.size[1], F.T[1], cy.seg, cy.lpf, lpl, lines, rpad, lds);
+    const int rc = pass_geometry(F.size[1], F.T[1], cy.seg, cy.lpf, lpl, lines, rpad, lds, kBanksNone);
     set_error(keep);
     if (rc || lines % 4) return false;
     const int64_t rstep = 2 * cy.lpf / (lines / 4);
@@ -734,7 +770,8 @@ int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<Pa
         int lpl = 1, lines = 1, rpad = 0, wgs = 0;
         size_t l = 0;
         rc = ch.kind == kPassPrefix ? prefix_geometry(L, F.T[axis], seg, ch.lpf, lpl, lines, l, wgs)
-                                    : pass_geometry(L, F.T[axis], seg, ch.lpf, lpl, lines, rpad, l);
+                                    : pass_geometry(L, F.T[axis], seg, ch.lpf, lpl, lines, rpad, l,
+                                                    axis == 0 ? kBanksRows : kBanksColumns);
         if (rc) return rc;
         const int nch = cg.Lc ? (int)((F.size[axis] + cg.C - 1) / cg.C) : 1;
         const int gkey = (ch.kind * 1024 + seg) * 1024 + ch.lpf;
@@ -1913,7 +1950,7 @@ int fb_accumulate_lines(int tail, int precision, int64_t nlines, int64_t len, in
         int lpl, lines, rpad;
         size_t lds;
         const int seg = model_seg(len, F.T[0]);
-        if ((rc = pass_geometry(len, F.T[0], seg, kBlock, lpl, lines, rpad, lds))) return rc;
+        if ((rc = pass_geometry(len, F.T[0], seg, kBlock, lpl, lines, rpad, lds, kBanksRows))) return rc;
         PassField pf{};
         pf.seg = seg;
         pf.len = len;
