@@ -1537,6 +1537,22 @@ int fb_plan_field_layout(fb_plan *P, int field, int *blocked)
     return FB_OK;
 }
 
+int fb_pass_layout(int64_t len, int T, int seg, int threads, int column_tiles, int *lanes_per_line, int *lines,
+                   int *gap, int64_t *lds_bytes)
+{
+    if (!lanes_per_line || !lines || !gap || !lds_bytes) return fail(FB_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (len < 1 || T < 0) return fail(FB_ERR_INVALID_ARGUMENT, "len must be >= 1 and T >= 0");
+    if (threads != 128 && threads != 256 && threads != 2 * kBlock)
+        return fail(FB_ERR_INVALID_ARGUMENT, "threads must be 128, 256 or 512");
+    if (!fast_pass_seg_supported(kPassSlide, seg))
+        return fail(FB_ERR_INVALID_ARGUMENT, "no pass instantiation for segment length " + std::to_string(seg));
+    size_t lds = 0;
+    const int rc = pass_geometry(len, T, seg, threads / 2, *lanes_per_line, *lines, *gap, lds,
+                                 column_tiles ? kBanksColumns : kBanksRows);
+    *lds_bytes = (int64_t)lds;
+    return rc;
+}
+
 int fb_plan_set_timing(fb_plan *P, int enable)
 {
     if (!P) return fail(FB_ERR_INVALID_ARGUMENT, "plan is NULL");

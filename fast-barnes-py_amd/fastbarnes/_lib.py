@@ -96,6 +96,9 @@ SIGNATURES = {
     'fb_plan_info': (ctypes.c_int, [_vp, _i64p, _i64p]),
     'fb_plan_field_layout': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     'fb_plan_set_blocked': (ctypes.c_int, [_vp, ctypes.c_int]),
+    'fb_pass_layout': (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                      ctypes.POINTER(ctypes.c_int), _i64p]),
     'fb_plan_destroy': (ctypes.c_int, [_vp]),
 }
 

@@ -250,6 +250,17 @@ int fb_plan_field_layout(fb_plan *plan, int field, int *blocked);
  * either way).  No reference counterpart. */
 int fb_plan_set_blocked(fb_plan *plan, int enable);
 
+/* LDS layout of the FP32C sliding pass over lines of length `len` with half kernel T, `seg`
+ * elements per lane (an instantiated segment length) and workgroups of `threads` threads
+ * (128, 256 or 512); column_tiles = 1 for a y or z pass, 0 for an x pass.  The workgroup's
+ * LDS holds T+1 zeros, then `lines` v lines and `lines` w lines at a stride of len + gap
+ * floats (each line's cells, then a gap whose last T+1 zeros are the next line's left
+ * padding), then a reserve of 64 floats; *lanes_per_line lanes take `seg` consecutive cells
+ * each.  *lds_bytes is the workgroup's LDS size.  Host code only (no device call): for tests
+ * of the layout's bounds.  No reference counterpart. */
+int fb_pass_layout(int64_t len, int T, int seg, int threads, int column_tiles, int *lanes_per_line, int *lines,
+                   int *gap, int64_t *lds_bytes);
+
 /* Per-stage device time (ms) of the last fb_plan_execute run with timing enabled,
  * measured with HIP events recorded on the execution stream.  names receives a
  * comma-separated stage list.  enable != 0 turns event recording on. */
