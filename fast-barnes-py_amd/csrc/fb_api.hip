@@ -486,6 +486,10 @@ struct fb_plan {
     int64_t *d_pt_prefix = nullptr, *d_xline_prefix = nullptr, *d_cell_prefix = nullptr;
     int64_t *d_lslot_prefix[3] = {nullptr, nullptr, nullptr};
     int64_t lslots[3] = {0, 0, 0};
+    // FP64 x lines through LDS (k_exact_rows): first workgroup of each field, workgroups in
+    // all, and the widest x kernel (2T+1) of the fields
+    int64_t *d_rowwg_prefix = nullptr;
+    int64_t rowwgs = 0, row_rect = 0;
     double *d_offsets = nullptr, *d_off_part = nullptr;
     int64_t max_n = 0;
     int64_t max_rows = 0;   // most x-lines of any field
@@ -884,7 +888,7 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
     P->corners = 1 << P->dim;
     const int fast = precision == FB_PREC_FP32C;
     P->fh.resize(nfields);
-    std::vector<int64_t> pt_prefix, xline_prefix, cell_prefix, lslot[3];
+    std::vector<int64_t> pt_prefix, xline_prefix, cell_prefix, lslot[3], rowwg;
     for (int f = 0; f < nfields; ++f) {
         int rc = make_field(fields[f], P->fh[f], fast);
         if (rc) { delete P; return rc; }
@@ -905,6 +909,9 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
             lslot[a].push_back(P->lslots[a]);
             if (a < P->dim) P->lslots[a] += 2 * nl[a];
         }
+        rowwg.push_back(P->rowwgs);
+        P->rowwgs += (2 * nl[0] + 63) / 64;
+        P->row_rect = std::max(P->row_rect, 2 * (int64_t)d.T[0] + 1);
         P->npoints += d.n;
         P->nxlines += H * D;
         // 256-byte aligned field starts (float4 tiles); an FP32C 2D field that may keep its
@@ -918,6 +925,7 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
     xline_prefix.push_back(P->nxlines);
     cell_prefix.push_back(P->ncells);
     for (int a = 0; a < 3; ++a) lslot[a].push_back(P->lslots[a]);
+    rowwg.push_back(P->rowwgs);
     P->nentries = P->npoints;   // one (base cell, point) pair per point
     if (P->ncells >= (int64_t)kInvalidKey || P->npoints >= (int64_t)UINT32_MAX || P->nentries >= (int64_t)UINT32_MAX) {
         delete P;
@@ -931,6 +939,7 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
     TRY(P->upload(&P->d_xline_prefix, xline_prefix));
     TRY(P->upload(&P->d_cell_prefix, cell_prefix));
     for (int a = 0; a < 3; ++a) TRY(P->upload(&P->d_lslot_prefix[a], lslot[a]));
+    TRY(P->upload(&P->d_rowwg_prefix, rowwg));
     TRY(P->alloc(&P->d_offsets, nfields));
     for (const FieldDev &d : P->fh) {
         P->max_n = std::max(P->max_n, d.n);
@@ -1041,9 +1050,13 @@ int plan_run(fb_plan *P, const double *pts, const double *val, float *out, float
         stage_mark(P, s, "inject");
         static const char *ln[3] = {"lines_x", "lines_y", "lines_z"};
         for (int a = 0; a < P->dim; ++a) {
-            if ((rc = launch_exact_lines(P->d_fields, P->d_lslot_prefix[a], P->nfields, P->lslots[a], a, -1, P->d_vg,
-                                         P->d_wg, P->d_vs, P->d_ws, s)))
-                return rc;
+            if (a == 0 && exact_rows_ring(P->row_rect))
+                rc = launch_exact_rows(P->d_fields, P->d_rowwg_prefix, P->d_lslot_prefix[0], P->nfields, P->rowwgs, -1,
+                                       P->row_rect, P->d_vg, P->d_wg, P->d_vs, P->d_ws, s);
+            else
+                rc = launch_exact_lines(P->d_fields, P->d_lslot_prefix[a], P->nfields, P->lslots[a], a, -1, P->d_vg,
+                                        P->d_wg, P->d_vs, P->d_ws, s);
+            if (rc) return rc;
             stage_mark(P, s, ln[a]);
         }
         if ((rc = launch_exact_finalize(P->d_fields, P->d_cell_prefix, P->nfields, P->ncells, P->d_vg, P->d_wg,
@@ -1948,12 +1961,18 @@ int fb_accumulate_lines(int tail, int precision, int64_t nlines, int64_t len, in
         double *d_w, *d_vs, *d_ws;
         int64_t *d_pre;
         if ((rc = sc.get(&d_w, cells)) || (rc = sc.get(&d_vs, cells)) || (rc = sc.get(&d_ws, cells)) ||
-            (rc = sc.get(&d_pre, 2)))
+            (rc = sc.get(&d_pre, 4)))
             return rc;
         if ((rc = launch_zero(d_w, cells * 8, s))) return rc;
-        const int64_t pre[2] = {0, 2 * nlines};
+        if ((rc = fast_pass_init())) return rc;   // k_exact_rows' LDS limit
+        // line slots, then the x-line workgroups of k_exact_rows
+        const int64_t nwg = (2 * nlines + 63) / 64;
+        const int64_t pre[4] = {0, 2 * nlines, 0, nwg};
         FB_HIP_CHECK(hipMemcpy(d_pre, pre, sizeof(pre), hipMemcpyHostToDevice));
-        if ((rc = launch_exact_lines(d_f, d_pre, 1, 2 * nlines, 0, rect_len, d_io, d_w, d_vs, d_ws, s))) return rc;
+        rc = exact_rows_ring(rect_len)
+                 ? launch_exact_rows(d_f, d_pre + 2, d_pre, 1, nwg, rect_len, rect_len, d_io, d_w, d_vs, d_ws, s)
+                 : launch_exact_lines(d_f, d_pre, 1, 2 * nlines, 0, rect_len, d_io, d_w, d_vs, d_ws, s);
+        if (rc) return rc;
     } else {
         if (rect_len % 2 == 0) return fail(FB_ERR_UNSUPPORTED, "FP32C line passes need an odd rect_len");
         if ((rc = fast_pass_init())) return rc;
@@ -2049,11 +2068,18 @@ int fb_convolve_grid(int tail, int precision, int dim, const int64_t *size, cons
     if (precision == FB_PREC_FP64) {
         double *d_vs, *d_ws;
         int64_t *d_pre;
-        if ((rc = sc.get(&d_vs, cells)) || (rc = sc.get(&d_ws, cells)) || (rc = sc.get(&d_pre, 2))) return rc;
+        if ((rc = sc.get(&d_vs, cells)) || (rc = sc.get(&d_ws, cells)) || (rc = sc.get(&d_pre, 4))) return rc;
+        if ((rc = fast_pass_init())) return rc;   // k_exact_rows' LDS limit
         for (int a = 0; a < dim; ++a) {
-            const int64_t pre[2] = {0, 2 * nl[a]};
+            // line slots, then (x lines) the workgroups of k_exact_rows
+            const int64_t nwg = (2 * nl[a] + 63) / 64;
+            const int64_t pre[4] = {0, 2 * nl[a], 0, nwg};
+            const int64_t rect = 2 * (int64_t)F.T[0] + 1;
             FB_HIP_CHECK(hipMemcpy(d_pre, pre, sizeof(pre), hipMemcpyHostToDevice));
-            if ((rc = launch_exact_lines(d_f, d_pre, 1, 2 * nl[a], a, -1, d_vg, d_wg, d_vs, d_ws, s))) return rc;
+            rc = a == 0 && exact_rows_ring(rect)
+                     ? launch_exact_rows(d_f, d_pre + 2, d_pre, 1, nwg, -1, rect, d_vg, d_wg, d_vs, d_ws, s)
+                     : launch_exact_lines(d_f, d_pre, 1, 2 * nl[a], a, -1, d_vg, d_wg, d_vs, d_ws, s);
+            if (rc) return rc;
             FB_HIP_CHECK(hipStreamSynchronize(s));   // d_pre is reused by the next axis
         }
         if ((rc = launch_mask(d_f, d_cpre, 1, cells, d_wg, 0, s))) return rc;

@@ -126,6 +126,24 @@ struct PointRec {
     int32_t pad_;
 };
 
+// FP64 x lines through LDS (k_exact_rows): tiles of kRowTile columns, a ring of `ring` tiles
+// per (line, field) slot, 64 slots per workgroup.  exact_rows_ring: the ring's tiles for a
+// kernel of rect terms (0 when the ring would not fit the LDS); exact_rows_lds: its bytes.
+constexpr int kRowTile = 32;
+__host__ __device__ inline int64_t exact_rows_slot_stride(int ring) { return (int64_t)ring * kRowTile + 1; }
+__host__ __device__ inline int64_t exact_rows_lds(int ring)
+{
+    return (64 * exact_rows_slot_stride(ring) + 64 * (kRowTile + 1)) * (int64_t)sizeof(double);
+}
+inline int exact_rows_ring(int64_t rect)
+{
+    const int64_t h0 = (rect - 1) / 2, h1 = rect - h0;
+    const int64_t need = h0 / kRowTile + (h1 + kRowTile - 1) / kRowTile + 2;
+    int ring = 1;
+    while (ring < need) ring <<= 1;
+    return exact_rows_lds(ring) <= 160 * 1024 ? ring : 0;
+}
+
 // Contribution record of the sorted injection list.
 struct Contrib {
     uint32_t key;          // global cell index (kInvalidKey when outside)

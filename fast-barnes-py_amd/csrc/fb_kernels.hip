@@ -620,6 +620,181 @@ __global__ __launch_bounds__(kBlock) void k_exact_lines(const FieldDev *__restri
     else exact_line<false>(G, S, L, st, rect, F.num_iter, F.tail, F.alpha[axis]);
 }
 
+// FP64 x lines through LDS.  One wave takes 32 consecutive x lines of one field, v and w each
+// (64 (line, field) slots, lane = slot).  Each iteration streams the slots' rows in tiles of
+// kRowTile columns: coalesced loads (two rows of 32 doubles per instruction) into a ring of
+// `ring` tiles per slot in LDS, one tile ahead of use, the line's steps from the ring in the
+// reference's order (exact_line's phases, interpolation.py:503-523), the outputs through an
+// LDS tile and coalesced stores into the other buffer (ping-pong, as exact_line).  One lane
+// runs one line's running sum exactly as one thread of k_exact_lines does, so the results are
+// the same bits; the rows are read 512 bytes per instruction instead of 64 lines of 8 bytes.
+// ring: a power of two >= h0/C + ceil(h1/C) + 2 tiles (C = kRowTile), which holds every term
+// the chunk of outputs [i*C, (i+1)*C) reads (tiles (i*C - h1)/C .. ((i+1)*C + h0)/C) while
+// the next tile loads.  Slot stride in LDS: ring*C + 1 doubles (odd: the lanes' reads of one
+// column fall on different banks).
+
+template <bool TAIL>
+__device__ __forceinline__ void exact_rows(const FieldDev &F, int64_t s0, int nsl, int rect_override, int ring,
+                                           double *__restrict__ rl, double *__restrict__ vg,
+                                           double *__restrict__ wg, double *__restrict__ vs,
+                                           double *__restrict__ ws)
+{
+    constexpr int C = kRowTile;
+    const int lane = threadIdx.x;
+    const int64_t L = F.size[0];
+    const int64_t rect = rect_override > 0 ? rect_override : 2 * (int64_t)F.T[0] + 1;
+    const int64_t h0 = (rect - 1) / 2, h0_1 = h0 + 1, h1 = rect - h0;
+    const double alpha = F.alpha[0];
+    const int mask = ring * C - 1;             // the ring holds ring*C columns: a power of two
+    const int64_t RS = exact_rows_slot_stride(ring);
+    double *ot = rl + 64 * RS;                 // output tile, slot stride C + 1
+    const int64_t NT = (L + C - 1) / C;
+    // loader / storer mapping: instruction j takes slot 2j + (lane >> 5), column lane & 31; the
+    // loads clamp row and column into the field (values past them are never used)
+    const int wl = lane >> 5, cl = lane & 31;
+    const int64_t line0 = s0 >> 1;
+    const int jmax = max(0, (nsl - 1 - wl) >> 1);
+    double *const bufG = (wl ? wg : vg) + F.cell_off + line0 * L;
+    double *const bufS = (wl ? ws : vs) + F.cell_off + line0 * L;
+    const double *my = rl + lane * RS;
+    const bool act = lane < nsl;
+
+    // the tile in flight: slot 2j + wl, column t*C + cl
+    double pa_[C];
+    auto fetch = [&](double (&pre)[C], const double *__restrict__ g, int64_t t) {
+        const int64_t c = min(t * C + cl, L - 1);
+#pragma unroll
+        for (int j = 0; j < C; ++j) pre[j] = g[(int64_t)min(j, jmax) * L + c];
+    };
+    auto put = [&](const double (&pre)[C], int64_t t) {
+        const int p = (int)(t * C + cl) & mask;
+#pragma unroll
+        for (int j = 0; j < C; ++j) rl[(2 * j + wl) * RS + p] = pre[j];
+    };
+    auto in_at = [&](int64_t p) { return my[(int)p & mask]; };
+    for (int it = 0; it < F.num_iter; ++it) {
+        const double *__restrict__ src = (it & 1) ? bufS : bufG;
+        double *__restrict__ dst = (it & 1) ? bufG : bufS;
+        // tiles the chunk of outputs [i*C, (i+1)*C) reads: up to ((i+1)*C + h0) / C; the ring
+        // receives one more tile after each chunk, loaded while the chunk runs
+        const int64_t need0 = min(NT - 1, (C + h0) / C);
+        for (int64_t t = 0; t <= need0; ++t) {
+            fetch(pa_, src, t);
+            put(pa_, t);
+        }
+        int64_t next = need0 + 1;   // the next tile to put
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        double accu = 0.0;
+        if (act)
+            for (int64_t k = -h0; k < 0; ++k) accu += in_at(k + h0);
+        auto chunk = [&](int64_t i) {
+            const int64_t k0 = i * C, k1 = min(k0 + C, L);
+            if (act) {
+                double *o = ot + lane * (C + 1);
+                if (k0 >= h1 && k0 + C <= L - (TAIL ? h0_1 : h0)) {
+                    // interior chunk: its terms into registers first (one LDS wait), then the
+                    // steps in the reference's order
+                    double a[C + 1], b[C];
+                    const int pa = (int)(k0 + h0), pb = (int)(k0 - h1);
+#pragma unroll
+                    for (int j = 0; j <= C; ++j) a[j] = my[(pa + j) & mask];
+#pragma unroll
+                    for (int j = 0; j < C; ++j) b[j] = my[(pb + j) & mask];
+#pragma unroll
+                    for (int j = 0; j < C; ++j) {
+                        accu += (a[j] - b[j]);
+                        o[j] = TAIL ? accu + alpha * (b[j] + a[j + 1]) : accu;
+                    }
+                } else {
+                    for (int64_t k = k0; k < k1; ++k) {
+                        double r;
+                        if (TAIL) {
+                            if (k < h1) {
+                                accu += in_at(k + h0);
+                                r = accu + alpha * in_at(k + h0_1);
+                            } else if (k < L - h0_1) {
+                                const double bk = in_at(k - h1);
+                                accu += (in_at(k + h0) - bk);
+                                r = accu + alpha * (bk + in_at(k + h0_1));
+                            } else if (k == L - h0_1) {
+                                const double bk = in_at(k - h1);
+                                accu += (in_at(k + h0) - bk);
+                                r = accu + alpha * bk;
+                            } else {
+                                const double bk = in_at(k - h1);
+                                accu -= bk;
+                                r = accu + alpha * bk;
+                            }
+                        } else {
+                            if (k < h1) accu += in_at(k + h0);
+                            else if (k < L - h0) accu += (in_at(k + h0) - in_at(k - h1));
+                            else accu -= in_at(k - h1);
+                            r = accu;
+                        }
+                        o[k - k0] = r;
+                    }
+                }
+            }
+        };
+        // the chunk's outputs to the other buffer (the storer mapping reads other lanes' rows)
+        auto store = [&](int64_t i) {
+            const int64_t c = i * C + cl;
+            double v[C];   // all LDS reads first (one wait), then the stores
+#pragma unroll
+            for (int j = 0; j < C; ++j) v[j] = ot[(2 * j + wl) * (C + 1) + cl];
+            if (c < L) {
+                if (nsl == 64) {   // every slot of the wave: stores without a branch each
+#pragma unroll
+                    for (int j = 0; j < C; ++j) dst[(int64_t)j * L + c] = v[j];
+                } else {
+#pragma unroll
+                    for (int j = 0; j < C; ++j)
+                        if (j <= jmax && 2 * j + wl < nsl) dst[(int64_t)j * L + c] = v[j];
+                }
+            }
+        };
+        // per chunk: the next tile's loads issued, the steps, the tile into the ring (its wait
+        // comes after the steps), the chunk's stores
+        for (int64_t i = 0; i < NT; ++i) {
+            const bool more = next < NT;
+            if (more) fetch(pa_, src, next);
+            chunk(i);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            if (more) put(pa_, next++);
+            store(i);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+    }
+    if (F.num_iter & 1) {
+        // the result is in S: copy it to G a tile at a time (each lane its own stores' addresses)
+        for (int64_t t = 0; t < NT; ++t) {
+            fetch(pa_, bufS, t);
+            const int64_t c = t * C + cl;
+#pragma unroll
+            for (int j = 0; j < C; ++j)
+                if (2 * j + wl < nsl && c < L) bufG[(int64_t)j * L + c] = pa_[j];
+        }
+    }
+}
+
+// One wave per workgroup: workgroup b takes slots 64*(b - wg_prefix[f]) .. of field f (slot
+// 2l + c: x line l, field c = v, w of the field's 2*H*D slots).
+__global__ __launch_bounds__(64) void k_exact_rows(const FieldDev *__restrict__ fields,
+                                                   const int64_t *__restrict__ wg_prefix,
+                                                   const int64_t *__restrict__ slot_prefix, int nfields,
+                                                   int rect_override, int ring, double *__restrict__ vg,
+                                                   double *__restrict__ wg, double *__restrict__ vs,
+                                                   double *__restrict__ ws)
+{
+    extern __shared__ double rl[];
+    const int f = find_seg(wg_prefix, nfields, blockIdx.x);
+    const FieldDev &F = fields[f];
+    const int64_t s0 = (blockIdx.x - wg_prefix[f]) * 64;
+    const int nsl = (int)min((int64_t)64, slot_prefix[f + 1] - slot_prefix[f] - s0);
+    if (F.tail) exact_rows<true>(F, s0, nsl, rect_override, ring, rl, vg, wg, vs, ws);
+    else exact_rows<false>(F, s0, nsl, rect_override, ring, rl, vg, wg, vs, ws);
+}
+
 // wg[wg < thr] = NaN, then (vg/wg + offset).astype(float32): interpolation.py:428-430, :367
 __global__ __launch_bounds__(kBlock) void k_exact_finalize(const FieldDev *__restrict__ fields,
                                                            const int64_t *__restrict__ cell_prefix, int nfields,
@@ -1897,6 +2072,22 @@ int launch_exact_lines(const FieldDev *d_fields, const int64_t *d_line_prefix, i
     return FB_OK;
 }
 
+int launch_exact_rows(const FieldDev *d_fields, const int64_t *d_wg_prefix, const int64_t *d_slot_prefix, int nfields,
+                      int64_t nwg, int rect_override, int64_t max_rect, double *d_vg, double *d_wg, double *d_vs,
+                      double *d_ws, hipStream_t s)
+{
+    if (nwg <= 0) return FB_OK;
+    const int ring = exact_rows_ring(max_rect);
+    if (!ring) {
+        set_error("FP64 x lines: kernel of " + std::to_string(max_rect) + " terms too wide for the LDS ring");
+        return FB_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(k_exact_rows, dim3((unsigned)nwg), dim3(64), (size_t)exact_rows_lds(ring), s, d_fields,
+                       d_wg_prefix, d_slot_prefix, nfields, rect_override, ring, d_vg, d_wg, d_vs, d_ws);
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
 int launch_exact_finalize(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfields, int64_t ncells,
                           const double *d_vg, const double *d_wg, const double *d_offsets, float *d_out,
                           hipStream_t s)
@@ -1942,6 +2133,9 @@ int fast_pass_init()
         if (err == hipSuccess)   // the point keys' row counters
             err = hipFuncSetAttribute((const void *)k_point_keys_hist, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       kKeysBinsMax * (int)sizeof(uint32_t));
+        if (err == hipSuccess)   // the FP64 x lines' rings
+            err = hipFuncSetAttribute((const void *)k_exact_rows, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      fast_pass_max_lds());
     });
     if (err != hipSuccess) {
         (void)hipGetLastError();
