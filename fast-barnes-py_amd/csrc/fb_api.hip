@@ -487,9 +487,10 @@ struct fb_plan {
     int64_t *d_lslot_prefix[3] = {nullptr, nullptr, nullptr};
     int64_t lslots[3] = {0, 0, 0};
     // FP64 x lines through LDS (k_exact_rows): first workgroup of each field, workgroups in
-    // all, and the widest x kernel (2T+1) of the fields
+    // all, slots per workgroup, and the widest x kernel (2T+1) of the fields
     int64_t *d_rowwg_prefix = nullptr;
     int64_t rowwgs = 0, row_rect = 0;
+    int row_slots = 64;
     double *d_offsets = nullptr, *d_off_part = nullptr;
     int64_t max_n = 0;
     int64_t max_rows = 0;   // most x-lines of any field
@@ -909,8 +910,6 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
             lslot[a].push_back(P->lslots[a]);
             if (a < P->dim) P->lslots[a] += 2 * nl[a];
         }
-        rowwg.push_back(P->rowwgs);
-        P->rowwgs += (2 * nl[0] + 63) / 64;
         P->row_rect = std::max(P->row_rect, 2 * (int64_t)d.T[0] + 1);
         P->npoints += d.n;
         P->nxlines += H * D;
@@ -925,6 +924,11 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
     xline_prefix.push_back(P->nxlines);
     cell_prefix.push_back(P->ncells);
     for (int a = 0; a < 3; ++a) lslot[a].push_back(P->lslots[a]);
+    P->row_slots = exact_rows_slots(P->lslots[0]);
+    for (const FieldDev &d : P->fh) {
+        rowwg.push_back(P->rowwgs);
+        P->rowwgs += (2 * d.size[1] * d.size[2] + P->row_slots - 1) / P->row_slots;
+    }
     rowwg.push_back(P->rowwgs);
     P->nentries = P->npoints;   // one (base cell, point) pair per point
     if (P->ncells >= (int64_t)kInvalidKey || P->npoints >= (int64_t)UINT32_MAX || P->nentries >= (int64_t)UINT32_MAX) {
@@ -1051,8 +1055,8 @@ int plan_run(fb_plan *P, const double *pts, const double *val, float *out, float
         static const char *ln[3] = {"lines_x", "lines_y", "lines_z"};
         for (int a = 0; a < P->dim; ++a) {
             if (a == 0 && exact_rows_ring(P->row_rect))
-                rc = launch_exact_rows(P->d_fields, P->d_rowwg_prefix, P->d_lslot_prefix[0], P->nfields, P->rowwgs, -1,
-                                       P->row_rect, P->d_vg, P->d_wg, P->d_vs, P->d_ws, s);
+                rc = launch_exact_rows(P->d_fields, P->d_rowwg_prefix, P->d_lslot_prefix[0], P->nfields, P->rowwgs,
+                                       P->row_slots, -1, P->row_rect, P->d_vg, P->d_wg, P->d_vs, P->d_ws, s);
             else
                 rc = launch_exact_lines(P->d_fields, P->d_lslot_prefix[a], P->nfields, P->lslots[a], a, -1, P->d_vg,
                                         P->d_wg, P->d_vs, P->d_ws, s);
@@ -1966,11 +1970,12 @@ int fb_accumulate_lines(int tail, int precision, int64_t nlines, int64_t len, in
         if ((rc = launch_zero(d_w, cells * 8, s))) return rc;
         if ((rc = fast_pass_init())) return rc;   // k_exact_rows' LDS limit
         // line slots, then the x-line workgroups of k_exact_rows
-        const int64_t nwg = (2 * nlines + 63) / 64;
+        const int sl = exact_rows_slots(2 * nlines);
+        const int64_t nwg = (2 * nlines + sl - 1) / sl;
         const int64_t pre[4] = {0, 2 * nlines, 0, nwg};
         FB_HIP_CHECK(hipMemcpy(d_pre, pre, sizeof(pre), hipMemcpyHostToDevice));
         rc = exact_rows_ring(rect_len)
-                 ? launch_exact_rows(d_f, d_pre + 2, d_pre, 1, nwg, rect_len, rect_len, d_io, d_w, d_vs, d_ws, s)
+                 ? launch_exact_rows(d_f, d_pre + 2, d_pre, 1, nwg, sl, rect_len, rect_len, d_io, d_w, d_vs, d_ws, s)
                  : launch_exact_lines(d_f, d_pre, 1, 2 * nlines, 0, rect_len, d_io, d_w, d_vs, d_ws, s);
         if (rc) return rc;
     } else {
@@ -2072,12 +2077,13 @@ int fb_convolve_grid(int tail, int precision, int dim, const int64_t *size, cons
         if ((rc = fast_pass_init())) return rc;   // k_exact_rows' LDS limit
         for (int a = 0; a < dim; ++a) {
             // line slots, then (x lines) the workgroups of k_exact_rows
-            const int64_t nwg = (2 * nl[a] + 63) / 64;
+            const int sl = exact_rows_slots(2 * nl[a]);
+            const int64_t nwg = (2 * nl[a] + sl - 1) / sl;
             const int64_t pre[4] = {0, 2 * nl[a], 0, nwg};
             const int64_t rect = 2 * (int64_t)F.T[0] + 1;
             FB_HIP_CHECK(hipMemcpy(d_pre, pre, sizeof(pre), hipMemcpyHostToDevice));
             rc = a == 0 && exact_rows_ring(rect)
-                     ? launch_exact_rows(d_f, d_pre + 2, d_pre, 1, nwg, -1, rect, d_vg, d_wg, d_vs, d_ws, s)
+                     ? launch_exact_rows(d_f, d_pre + 2, d_pre, 1, nwg, sl, -1, rect, d_vg, d_wg, d_vs, d_ws, s)
                      : launch_exact_lines(d_f, d_pre, 1, 2 * nl[a], a, -1, d_vg, d_wg, d_vs, d_ws, s);
             if (rc) return rc;
             FB_HIP_CHECK(hipStreamSynchronize(s));   // d_pre is reused by the next axis

@@ -127,13 +127,14 @@ struct PointRec {
 };
 
 // FP64 x lines through LDS (k_exact_rows): tiles of kRowTile columns, a ring of `ring` tiles
-// per (line, field) slot, 64 slots per workgroup.  exact_rows_ring: the ring's tiles for a
-// kernel of rect terms (0 when the ring would not fit the LDS); exact_rows_lds: its bytes.
+// per (line, field) slot, 32 or 64 slots per workgroup.  exact_rows_ring: the ring's tiles for
+// a kernel of rect terms (0 when a 64-slot workgroup's ring would not fit the LDS);
+// exact_rows_lds: the bytes of a workgroup of `slots` slots.
 constexpr int kRowTile = 32;
 __host__ __device__ inline int64_t exact_rows_slot_stride(int ring) { return (int64_t)ring * kRowTile + 1; }
-__host__ __device__ inline int64_t exact_rows_lds(int ring)
+__host__ __device__ inline int64_t exact_rows_lds(int ring, int slots)
 {
-    return (64 * exact_rows_slot_stride(ring) + 64 * (kRowTile + 1)) * (int64_t)sizeof(double);
+    return (slots * exact_rows_slot_stride(ring) + slots * (kRowTile + 1)) * (int64_t)sizeof(double);
 }
 inline int exact_rows_ring(int64_t rect)
 {
@@ -141,8 +142,11 @@ inline int exact_rows_ring(int64_t rect)
     const int64_t need = h0 / kRowTile + (h1 + kRowTile - 1) / kRowTile + 2;
     int ring = 1;
     while (ring < need) ring <<= 1;
-    return exact_rows_lds(ring) <= 160 * 1024 ? ring : 0;
+    return exact_rows_lds(ring, 64) <= 160 * 1024 ? ring : 0;
 }
+// slots per workgroup of k_exact_rows for a plan of `slots` x-line slots: 32 when 64-slot
+// workgroups would leave most CUs of the GPU idle
+inline int exact_rows_slots(int64_t slots) { return (slots + 63) / 64 < 256 ? 32 : 64; }
 
 // Contribution record of the sorted injection list.
 struct Contrib {

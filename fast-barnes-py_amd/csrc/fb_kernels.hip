@@ -435,13 +435,16 @@ __device__ __forceinline__ void line_streams(const FieldDev &F, int64_t line, bo
 // One thread per x-line: merges the line's sub-streams by (x, point index) and adds
 // w*val_c and w to the FP64 grids in that order, which is the per-cell order of the
 // reference's point loop (each cell gets its contributions in increasing point index).
-__global__ __launch_bounds__(kBlock) void k_exact_inject(
+// One wave per workgroup, as k_exact_lines: a field has only as many threads as x lines.
+constexpr int kExactInjectThreads = 64;
+
+__global__ __launch_bounds__(kExactInjectThreads) void k_exact_inject(
     const FieldDev *__restrict__ fields, const int64_t *__restrict__ xline_prefix, int nfields, int64_t nxlines,
     const uint32_t *__restrict__ vals, const uint32_t *__restrict__ row_start,
     const uint32_t *__restrict__ row_count, const double *__restrict__ pts, const double *__restrict__ val,
     const double *__restrict__ offsets, double *__restrict__ vg, double *__restrict__ wg)
 {
-    const int64_t gl = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t gl = (int64_t)blockIdx.x * kExactInjectThreads + threadIdx.x;
     if (gl >= nxlines) return;
     const int f = find_seg(xline_prefix, nfields, gl);
     const FieldDev &F = fields[f];
@@ -591,13 +594,17 @@ __device__ void exact_line(double *__restrict__ G, double *__restrict__ S, int64
 #undef H_
 }
 
-__global__ __launch_bounds__(kBlock) void k_exact_lines(const FieldDev *__restrict__ fields,
+// One wave per workgroup: a field's few thousand line chains then spread over as many CUs as
+// they have waves instead of sharing a quarter as many CUs' memory pipelines.
+constexpr int kExactLineThreads = 64;
+
+__global__ __launch_bounds__(kExactLineThreads) void k_exact_lines(const FieldDev *__restrict__ fields,
                                                         const int64_t *__restrict__ line_prefix, int nfields,
                                                         int64_t nslots, int axis, int rect_override,
                                                         double *__restrict__ vg, double *__restrict__ wg,
                                                         double *__restrict__ vs, double *__restrict__ ws)
 {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t g = (int64_t)blockIdx.x * kExactLineThreads + threadIdx.x;
     if (g >= nslots) return;
     const int f = find_seg(line_prefix, nfields, g);
     const FieldDev &F = fields[f];
@@ -633,13 +640,14 @@ __global__ __launch_bounds__(kBlock) void k_exact_lines(const FieldDev *__restri
 // the next tile loads.  Slot stride in LDS: ring*C + 1 doubles (odd: the lanes' reads of one
 // column fall on different banks).
 
-template <bool TAIL>
+template <bool TAIL, int S>
 __device__ __forceinline__ void exact_rows(const FieldDev &F, int64_t s0, int nsl, int rect_override, int ring,
                                            double *__restrict__ rl, double *__restrict__ vg,
                                            double *__restrict__ wg, double *__restrict__ vs,
                                            double *__restrict__ ws)
 {
     constexpr int C = kRowTile;
+    constexpr int NP = S / 2;   // slot pairs: a load or store instruction takes two rows
     const int lane = threadIdx.x;
     const int64_t L = F.size[0];
     const int64_t rect = rect_override > 0 ? rect_override : 2 * (int64_t)F.T[0] + 1;
@@ -647,7 +655,7 @@ __device__ __forceinline__ void exact_rows(const FieldDev &F, int64_t s0, int ns
     const double alpha = F.alpha[0];
     const int mask = ring * C - 1;             // the ring holds ring*C columns: a power of two
     const int64_t RS = exact_rows_slot_stride(ring);
-    double *ot = rl + 64 * RS;                 // output tile, slot stride C + 1
+    double *ot = rl + S * RS;                  // output tile, slot stride C + 1
     const int64_t NT = (L + C - 1) / C;
     // loader / storer mapping: instruction j takes slot 2j + (lane >> 5), column lane & 31; the
     // loads clamp row and column into the field (values past them are never used)
@@ -660,16 +668,16 @@ __device__ __forceinline__ void exact_rows(const FieldDev &F, int64_t s0, int ns
     const bool act = lane < nsl;
 
     // the tile in flight: slot 2j + wl, column t*C + cl
-    double pa_[C];
-    auto fetch = [&](double (&pre)[C], const double *__restrict__ g, int64_t t) {
+    double pa_[NP];
+    auto fetch = [&](double (&pre)[NP], const double *__restrict__ g, int64_t t) {
         const int64_t c = min(t * C + cl, L - 1);
 #pragma unroll
-        for (int j = 0; j < C; ++j) pre[j] = g[(int64_t)min(j, jmax) * L + c];
+        for (int j = 0; j < NP; ++j) pre[j] = g[(int64_t)min(j, jmax) * L + c];
     };
-    auto put = [&](const double (&pre)[C], int64_t t) {
+    auto put = [&](const double (&pre)[NP], int64_t t) {
         const int p = (int)(t * C + cl) & mask;
 #pragma unroll
-        for (int j = 0; j < C; ++j) rl[(2 * j + wl) * RS + p] = pre[j];
+        for (int j = 0; j < NP; ++j) rl[(2 * j + wl) * RS + p] = pre[j];
     };
     auto in_at = [&](int64_t p) { return my[(int)p & mask]; };
     for (int it = 0; it < F.num_iter; ++it) {
@@ -739,16 +747,16 @@ __device__ __forceinline__ void exact_rows(const FieldDev &F, int64_t s0, int ns
         // the chunk's outputs to the other buffer (the storer mapping reads other lanes' rows)
         auto store = [&](int64_t i) {
             const int64_t c = i * C + cl;
-            double v[C];   // all LDS reads first (one wait), then the stores
+            double v[NP];   // all LDS reads first (one wait), then the stores
 #pragma unroll
-            for (int j = 0; j < C; ++j) v[j] = ot[(2 * j + wl) * (C + 1) + cl];
+            for (int j = 0; j < NP; ++j) v[j] = ot[(2 * j + wl) * (C + 1) + cl];
             if (c < L) {
-                if (nsl == 64) {   // every slot of the wave: stores without a branch each
+                if (nsl == S) {   // every slot of the workgroup: stores without a branch each
 #pragma unroll
-                    for (int j = 0; j < C; ++j) dst[(int64_t)j * L + c] = v[j];
+                    for (int j = 0; j < NP; ++j) dst[(int64_t)j * L + c] = v[j];
                 } else {
 #pragma unroll
-                    for (int j = 0; j < C; ++j)
+                    for (int j = 0; j < NP; ++j)
                         if (j <= jmax && 2 * j + wl < nsl) dst[(int64_t)j * L + c] = v[j];
                 }
             }
@@ -771,14 +779,16 @@ __device__ __forceinline__ void exact_rows(const FieldDev &F, int64_t s0, int ns
             fetch(pa_, bufS, t);
             const int64_t c = t * C + cl;
 #pragma unroll
-            for (int j = 0; j < C; ++j)
+            for (int j = 0; j < NP; ++j)
                 if (2 * j + wl < nsl && c < L) bufG[(int64_t)j * L + c] = pa_[j];
         }
     }
 }
 
-// One wave per workgroup: workgroup b takes slots 64*(b - wg_prefix[f]) .. of field f (slot
-// 2l + c: x line l, field c = v, w of the field's 2*H*D slots).
+// One wave per workgroup: workgroup b takes the S slots from S*(b - wg_prefix[f]) on of field
+// f (slot 2l + c: x line l, field c = v, w of the field's 2*H*D slots).  S = 32 (lanes 32-63
+// load and store, but run no line) spreads the few lines of a small plan over twice the CUs.
+template <int S>
 __global__ __launch_bounds__(64) void k_exact_rows(const FieldDev *__restrict__ fields,
                                                    const int64_t *__restrict__ wg_prefix,
                                                    const int64_t *__restrict__ slot_prefix, int nfields,
@@ -789,10 +799,10 @@ __global__ __launch_bounds__(64) void k_exact_rows(const FieldDev *__restrict__ 
     extern __shared__ double rl[];
     const int f = find_seg(wg_prefix, nfields, blockIdx.x);
     const FieldDev &F = fields[f];
-    const int64_t s0 = (blockIdx.x - wg_prefix[f]) * 64;
-    const int nsl = (int)min((int64_t)64, slot_prefix[f + 1] - slot_prefix[f] - s0);
-    if (F.tail) exact_rows<true>(F, s0, nsl, rect_override, ring, rl, vg, wg, vs, ws);
-    else exact_rows<false>(F, s0, nsl, rect_override, ring, rl, vg, wg, vs, ws);
+    const int64_t s0 = (blockIdx.x - wg_prefix[f]) * S;
+    const int nsl = (int)min((int64_t)S, slot_prefix[f + 1] - slot_prefix[f] - s0);
+    if (F.tail) exact_rows<true, S>(F, s0, nsl, rect_override, ring, rl, vg, wg, vs, ws);
+    else exact_rows<false, S>(F, s0, nsl, rect_override, ring, rl, vg, wg, vs, ws);
 }
 
 // wg[wg < thr] = NaN, then (vg/wg + offset).astype(float32): interpolation.py:428-430, :367
@@ -2056,8 +2066,9 @@ int launch_exact_inject(const FieldDev *d_fields, const int64_t *d_xline_prefix,
                         double *d_wg, hipStream_t s)
 {
     if (nxlines <= 0) return FB_OK;
-    hipLaunchKernelGGL(k_exact_inject, dim3(nblocks(nxlines)), dim3(kBlock), 0, s, d_fields, d_xline_prefix,
-                       nfields, nxlines, d_vals, d_row_start, d_row_count, d_pts, d_val, d_offsets, d_vg, d_wg);
+    hipLaunchKernelGGL(k_exact_inject, dim3((unsigned)((nxlines + kExactInjectThreads - 1) / kExactInjectThreads)),
+                       dim3(kExactInjectThreads), 0, s, d_fields, d_xline_prefix, nfields, nxlines, d_vals,
+                       d_row_start, d_row_count, d_pts, d_val, d_offsets, d_vg, d_wg);
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }
@@ -2066,15 +2077,16 @@ int launch_exact_lines(const FieldDev *d_fields, const int64_t *d_line_prefix, i
                        int axis, int rect_len, double *d_vg, double *d_wg, double *d_vs, double *d_ws, hipStream_t s)
 {
     if (nslots <= 0) return FB_OK;
-    hipLaunchKernelGGL(k_exact_lines, dim3(nblocks(nslots)), dim3(kBlock), 0, s, d_fields, d_line_prefix, nfields,
-                       nslots, axis, rect_len, d_vg, d_wg, d_vs, d_ws);
+    hipLaunchKernelGGL(k_exact_lines, dim3((unsigned)((nslots + kExactLineThreads - 1) / kExactLineThreads)),
+                       dim3(kExactLineThreads), 0, s, d_fields, d_line_prefix, nfields, nslots, axis, rect_len, d_vg,
+                       d_wg, d_vs, d_ws);
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }
 
 int launch_exact_rows(const FieldDev *d_fields, const int64_t *d_wg_prefix, const int64_t *d_slot_prefix, int nfields,
-                      int64_t nwg, int rect_override, int64_t max_rect, double *d_vg, double *d_wg, double *d_vs,
-                      double *d_ws, hipStream_t s)
+                      int64_t nwg, int slots_per_wg, int rect_override, int64_t max_rect, double *d_vg, double *d_wg,
+                      double *d_vs, double *d_ws, hipStream_t s)
 {
     if (nwg <= 0) return FB_OK;
     const int ring = exact_rows_ring(max_rect);
@@ -2082,8 +2094,12 @@ int launch_exact_rows(const FieldDev *d_fields, const int64_t *d_wg_prefix, cons
         set_error("FP64 x lines: kernel of " + std::to_string(max_rect) + " terms too wide for the LDS ring");
         return FB_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL(k_exact_rows, dim3((unsigned)nwg), dim3(64), (size_t)exact_rows_lds(ring), s, d_fields,
-                       d_wg_prefix, d_slot_prefix, nfields, rect_override, ring, d_vg, d_wg, d_vs, d_ws);
+    if (slots_per_wg == 32)
+        hipLaunchKernelGGL(k_exact_rows<32>, dim3((unsigned)nwg), dim3(64), (size_t)exact_rows_lds(ring, 32), s,
+                           d_fields, d_wg_prefix, d_slot_prefix, nfields, rect_override, ring, d_vg, d_wg, d_vs, d_ws);
+    else
+        hipLaunchKernelGGL(k_exact_rows<64>, dim3((unsigned)nwg), dim3(64), (size_t)exact_rows_lds(ring, 64), s,
+                           d_fields, d_wg_prefix, d_slot_prefix, nfields, rect_override, ring, d_vg, d_wg, d_vs, d_ws);
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }
@@ -2134,7 +2150,10 @@ int fast_pass_init()
             err = hipFuncSetAttribute((const void *)k_point_keys_hist, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       kKeysBinsMax * (int)sizeof(uint32_t));
         if (err == hipSuccess)   // the FP64 x lines' rings
-            err = hipFuncSetAttribute((const void *)k_exact_rows, hipFuncAttributeMaxDynamicSharedMemorySize,
+            err = hipFuncSetAttribute((const void *)k_exact_rows<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      fast_pass_max_lds());
+        if (err == hipSuccess)
+            err = hipFuncSetAttribute((const void *)k_exact_rows<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       fast_pass_max_lds());
     });
     if (err != hipSuccess) {

@@ -60,13 +60,14 @@ int launch_exact_lines(const FieldDev *d_fields, const int64_t *d_line_prefix, i
                        int axis, int rect_len, double *d_vg, double *d_wg, double *d_vs, double *d_ws,
                        hipStream_t stream);
 // FP64 x lines through LDS (k_exact_rows), the same bits as launch_exact_lines along x:
-// d_wg_prefix[f] = first workgroup of field f (ceil(2*H*D/64) each, nwg in all), d_slot_prefix
+// d_wg_prefix[f] = first workgroup of field f (ceil(2*H*D/slots_per_wg) each, nwg in all;
+// slots_per_wg = exact_rows_slots of the plan's slots: 32 or 64), d_slot_prefix
 // the line slots as for launch_exact_lines; max_rect = the widest kernel (2T+1 or
 // rect_override) of the fields.  FB_ERR_UNSUPPORTED when its ring would not fit the LDS
 // (exact_rows_ring(max_rect) == 0): take launch_exact_lines then.
 int launch_exact_rows(const FieldDev *d_fields, const int64_t *d_wg_prefix, const int64_t *d_slot_prefix, int nfields,
-                      int64_t nwg, int rect_override, int64_t max_rect, double *d_vg, double *d_wg, double *d_vs,
-                      double *d_ws, hipStream_t stream);
+                      int64_t nwg, int slots_per_wg, int rect_override, int64_t max_rect, double *d_vg, double *d_wg,
+                      double *d_vs, double *d_ws, hipStream_t stream);
 int launch_exact_finalize(const FieldDev *d_fields, const int64_t *d_cell_prefix, int nfields, int64_t ncells,
                           const double *d_vg, const double *d_wg, const double *d_offsets, float *d_out,
                           hipStream_t stream);
