@@ -577,6 +577,30 @@ struct fb_plan {
         *p = (T *)q;
         return FB_OK;
     }
+    // several host arrays into one device block with one copy, each 256-byte aligned (a plan's
+    // descriptors and prefixes: one synchronous copy instead of one per array)
+    struct Upload {
+        void **dst;
+        const void *src;
+        size_t bytes;
+    };
+    int upload_all(const std::vector<Upload> &ups)
+    {
+        size_t total = 0;
+        for (const Upload &u : ups) total += round_bytes(u.bytes ? u.bytes : 1);
+        char *d = nullptr;
+        int rc = alloc(&d, total);
+        if (rc) return rc;
+        std::vector<char> h(total);
+        size_t off = 0;
+        for (const Upload &u : ups) {
+            if (u.bytes) std::memcpy(h.data() + off, u.src, u.bytes);
+            *u.dst = d + off;
+            off += round_bytes(u.bytes ? u.bytes : 1);
+        }
+        FB_HIP_CHECK(hipMemcpy(d, h.data(), total, hipMemcpyHostToDevice));
+        return FB_OK;
+    }
     template <class T>
     int upload(T **p, const std::vector<T> &h)
     {
@@ -841,15 +865,16 @@ int build_pass(fb_plan *P, int axis, std::vector<PassWork> &work, std::vector<Pa
 // (Re)builds and uploads the fast passes of all axes of an FP32C plan.
 int build_passes(fb_plan *P)
 {
+    std::vector<PassWork> work[3];
+    std::vector<PassField> pf[3];
+    std::vector<fb_plan::Upload> ups;
     for (int a = 0; a < P->dim; ++a) {
-        std::vector<PassWork> work;
-        std::vector<PassField> pf;
-        int rc = build_pass(P, a, work, pf, P->groups[a]);
+        const int rc = build_pass(P, a, work[a], pf[a], P->groups[a]);
         if (rc) return rc;
-        if ((rc = P->upload(&P->d_work[a], work))) return rc;
-        if ((rc = P->upload(&P->d_pass[a], pf))) return rc;
+        ups.push_back({reinterpret_cast<void **>(&P->d_work[a]), work[a].data(), work[a].size() * sizeof(PassWork)});
+        ups.push_back({reinterpret_cast<void **>(&P->d_pass[a]), pf[a].data(), pf[a].size() * sizeof(PassField)});
     }
-    return FB_OK;
+    return P->upload_all(ups);   // one copy for all axes
 }
 
 // One fast pass along `axis` over the whole batch: a launch per segment-length group.
@@ -938,12 +963,15 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
 
     int rc = FB_OK;
 #define TRY(x) do { rc = (x); if (rc) { delete P; return rc; } } while (0)
-    TRY(P->upload(&P->d_fields, P->fh));
-    TRY(P->upload(&P->d_pt_prefix, pt_prefix));
-    TRY(P->upload(&P->d_xline_prefix, xline_prefix));
-    TRY(P->upload(&P->d_cell_prefix, cell_prefix));
-    for (int a = 0; a < 3; ++a) TRY(P->upload(&P->d_lslot_prefix[a], lslot[a]));
-    TRY(P->upload(&P->d_rowwg_prefix, rowwg));
+    {
+        auto item = [](auto **dst, const auto &v) {
+            return fb_plan::Upload{reinterpret_cast<void **>(dst), v.data(), v.size() * sizeof(v[0])};
+        };
+        TRY(P->upload_all({item(&P->d_fields, P->fh), item(&P->d_pt_prefix, pt_prefix),
+                           item(&P->d_xline_prefix, xline_prefix), item(&P->d_cell_prefix, cell_prefix),
+                           item(&P->d_lslot_prefix[0], lslot[0]), item(&P->d_lslot_prefix[1], lslot[1]),
+                           item(&P->d_lslot_prefix[2], lslot[2]), item(&P->d_rowwg_prefix, rowwg)}));
+    }
     TRY(P->alloc(&P->d_offsets, nfields));
     for (const FieldDev &d : P->fh) {
         P->max_n = std::max(P->max_n, d.n);
