@@ -51,6 +51,34 @@ def test_batch_equals_single_calls(precision):
     assert info['grid_points'] == sum(c['size'][0] * c['size'][1] for c in fs)
 
 
+def test_fp64_x_line_workgroup_sizes_agree(orc):
+    """The FP64 x lines run through an LDS ring, one wave per 32 (line, field) slots for small
+    plans and per 64 for plans of at least 16384 slots.  A batch of 7 fields of 1200x1200
+    (16800 slots: 64-slot workgroups, the last of each field partial) must equal the
+    single-field runs (2400 slots: 32-slot workgroups) bit for bit, with odd and even iteration
+    counts and both methods, and a field must equal the oracle."""
+    from fastbarnes import workloads
+    from fastbarnes.batched import BarnesBatch
+    from fastbarnes.interpolation import barnes
+    rng = np.random.default_rng(5)
+    fs = []
+    for i in range(7):
+        pts = workloads.stations_2d(int(rng.integers(1000, 3000)), 700 + i)
+        fs.append(dict(pts=pts, val=workloads.values_sinusoids(pts, i), sigma=float(rng.uniform(0.6, 1.6)),
+                       x0=np.array([-26.0, 34.0]), step=75.0 / 1200, size=(1200, 1200), num_iter=3 + i % 4,
+                       max_dist=3.5, method='convolution' if i == 3 else 'optimized_convolution'))
+    descs = [dict(_desc(c), method=c['method']) for c in fs]
+    outs = BarnesBatch(descs, precision='fp64').interpolate([c['pts'] for c in fs], [c['val'] for c in fs])
+    for c, o in zip(fs, outs):
+        ref = barnes(c['pts'], c['val'], c['sigma'], c['x0'], c['step'], c['size'], method=c['method'],
+                     num_iter=c['num_iter'], max_dist=c['max_dist'], precision='fp64')
+        _bitwise(o, ref)
+    c = fs[1]
+    ref = orc.interpolate_convol(c['pts'], c['val'], c['sigma'], c['x0'], c['step'], c['size'], num_iter=c['num_iter'],
+                                 max_dist=c['max_dist'])
+    _bitwise(outs[1], ref)
+
+
 def test_graph_replays_match_eager():
     from fastbarnes import hip
     from fastbarnes.batched import BarnesBatch
