@@ -144,9 +144,14 @@ inline int exact_rows_ring(int64_t rect)
     while (ring < need) ring <<= 1;
     return exact_rows_lds(ring, 64) <= 160 * 1024 ? ring : 0;
 }
-// slots per workgroup of k_exact_rows for a plan of `slots` x-line slots: 32 when 64-slot
-// workgroups would leave most CUs of the GPU idle
-inline int exact_rows_slots(int64_t slots) { return (slots + 63) / 64 < 256 ? 32 : 64; }
+// slots per workgroup of k_exact_rows for a plan of `slots` x-line slots: 64, or 32 or 16
+// while fewer than 256 workgroups (one per CU of an MI355X) would take the lines
+inline int exact_rows_slots(int64_t slots)
+{
+    int s = 64;
+    while (s > 16 && (slots + s - 1) / s < 256) s >>= 1;
+    return s;
+}
 
 // Contribution record of the sorted injection list.
 struct Contrib {

@@ -786,8 +786,9 @@ __device__ __forceinline__ void exact_rows(const FieldDev &F, int64_t s0, int ns
 }
 
 // One wave per workgroup: workgroup b takes the S slots from S*(b - wg_prefix[f]) on of field
-// f (slot 2l + c: x line l, field c = v, w of the field's 2*H*D slots).  S = 32 (lanes 32-63
-// load and store, but run no line) spreads the few lines of a small plan over twice the CUs.
+// f (slot 2l + c: x line l, field c = v, w of the field's 2*H*D slots).  S = 32 or 16 (lanes
+// past S load and store, but run no line) spreads the few lines of a small plan over more CUs,
+// each chunk then moving a half or a quarter of the bytes.
 template <int S>
 __global__ __launch_bounds__(64) void k_exact_rows(const FieldDev *__restrict__ fields,
                                                    const int64_t *__restrict__ wg_prefix,
@@ -2094,7 +2095,10 @@ int launch_exact_rows(const FieldDev *d_fields, const int64_t *d_wg_prefix, cons
         set_error("FP64 x lines: kernel of " + std::to_string(max_rect) + " terms too wide for the LDS ring");
         return FB_ERR_UNSUPPORTED;
     }
-    if (slots_per_wg == 32)
+    if (slots_per_wg == 16)
+        hipLaunchKernelGGL(k_exact_rows<16>, dim3((unsigned)nwg), dim3(64), (size_t)exact_rows_lds(ring, 16), s,
+                           d_fields, d_wg_prefix, d_slot_prefix, nfields, rect_override, ring, d_vg, d_wg, d_vs, d_ws);
+    else if (slots_per_wg == 32)
         hipLaunchKernelGGL(k_exact_rows<32>, dim3((unsigned)nwg), dim3(64), (size_t)exact_rows_lds(ring, 32), s,
                            d_fields, d_wg_prefix, d_slot_prefix, nfields, rect_override, ring, d_vg, d_wg, d_vs, d_ws);
     else
@@ -2154,6 +2158,9 @@ int fast_pass_init()
                                       fast_pass_max_lds());
         if (err == hipSuccess)
             err = hipFuncSetAttribute((const void *)k_exact_rows<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      fast_pass_max_lds());
+        if (err == hipSuccess)
+            err = hipFuncSetAttribute((const void *)k_exact_rows<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       fast_pass_max_lds());
     });
     if (err != hipSuccess) {

@@ -52,11 +52,11 @@ def test_batch_equals_single_calls(precision):
 
 
 def test_fp64_x_line_workgroup_sizes_agree(orc):
-    """The FP64 x lines run through an LDS ring, one wave per 32 (line, field) slots for small
-    plans and per 64 for plans of at least 16384 slots.  A batch of 7 fields of 1200x1200
-    (16800 slots: 64-slot workgroups, the last of each field partial) must equal the
-    single-field runs (2400 slots: 32-slot workgroups) bit for bit, with odd and even iteration
-    counts and both methods, and a field must equal the oracle."""
+    """The FP64 x lines run through an LDS ring, one wave per 16, 32 or 64 (line, field) slots:
+    64 for plans of at least 16384 slots, 32 from 8192, 16 below.  A batch of 7 fields of
+    1200x1200 (16800 slots: 64-slot workgroups, the last of each field partial) and one of 4 of
+    them (9600 slots: 32) must equal the single-field runs (2400 slots: 16) bit for bit, with odd
+    and even iteration counts and both methods, and a field must equal the oracle."""
     from fastbarnes import workloads
     from fastbarnes.batched import BarnesBatch
     from fastbarnes.interpolation import barnes
@@ -69,10 +69,14 @@ def test_fp64_x_line_workgroup_sizes_agree(orc):
                        max_dist=3.5, method='convolution' if i == 3 else 'optimized_convolution'))
     descs = [dict(_desc(c), method=c['method']) for c in fs]
     outs = BarnesBatch(descs, precision='fp64').interpolate([c['pts'] for c in fs], [c['val'] for c in fs])
-    for c, o in zip(fs, outs):
+    outs4 = BarnesBatch(descs[:4], precision='fp64').interpolate([c['pts'] for c in fs[:4]],
+                                                                  [c['val'] for c in fs[:4]])
+    for i, (c, o) in enumerate(zip(fs, outs)):
         ref = barnes(c['pts'], c['val'], c['sigma'], c['x0'], c['step'], c['size'], method=c['method'],
                      num_iter=c['num_iter'], max_dist=c['max_dist'], precision='fp64')
         _bitwise(o, ref)
+        if i < 4:
+            _bitwise(outs4[i], ref)
     c = fs[1]
     ref = orc.interpolate_convol(c['pts'], c['val'], c['sigma'], c['x0'], c['step'], c['size'], num_iter=c['num_iter'],
                                  max_dist=c['max_dist'])
