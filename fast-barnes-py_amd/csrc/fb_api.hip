@@ -1077,7 +1077,7 @@ int plan_run(fb_plan *P, const double *pts, const double *val, float *out, float
         if ((rc = launch_zero(P->d_vg, P->ncells * 8, s))) return rc;
         if ((rc = launch_zero(P->d_wg, P->ncells * 8, s))) return rc;
         if ((rc = launch_exact_inject(P->d_fields, P->d_xline_prefix, P->nfields, P->nxlines, vs, P->d_line_start,
-                                      P->d_line_count, pts, val, P->d_offsets, P->d_vg, P->d_wg, s)))
+                                      P->d_line_count, pts, val, P->d_offsets, P->d_vg, P->d_wg, true, s)))
             return rc;
         stage_mark(P, s, "inject");
         static const char *ln[3] = {"lines_x", "lines_y", "lines_z"};
@@ -1827,7 +1827,7 @@ int fb_inject(int dim, int64_t n, const double *pts, const double *val, const do
     }
     if (!rc)
         rc = launch_exact_inject(P->d_fields, P->d_xline_prefix, 1, P->nxlines, vs, P->d_line_start,
-                                 P->d_line_count, d_pts, d_val, P->d_offsets, d_vg, d_wg, s);
+                                 P->d_line_count, d_pts, d_val, P->d_offsets, d_vg, d_wg, false, s);
     if (!rc && !on_dev) {
         if (hipMemcpyAsync(vg, d_vg, cells * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipMemcpyAsync(wg, d_wg, cells * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) {

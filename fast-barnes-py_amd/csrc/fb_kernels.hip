@@ -435,71 +435,208 @@ __device__ __forceinline__ void line_streams(const FieldDev &F, int64_t line, bo
 // One thread per x-line: merges the line's sub-streams by (x, point index) and adds
 // w*val_c and w to the FP64 grids in that order, which is the per-cell order of the
 // reference's point loop (each cell gets its contributions in increasing point index).
-// One wave per workgroup, as k_exact_lines: a field has only as many threads as x lines.
+// The contributions of a cell are consecutive in the merged order, so a cell's two sums are
+// held in registers and stored once; they start at 0.0 when the grids were cleared (zeroed),
+// else at the grids' values.  One wave per workgroup, as k_exact_lines: a field has only as
+// many threads as x lines.
+//
+// The merge is a chain of dependent steps.  When a workgroup's lines lie in one 1D or 2D
+// field and their rows (each line's own and, in 2D, the row below) hold at most
+// kInjectStage points, the wave first stages those points' records in LDS, all of their
+// loads and divisions in parallel, and the merge reads its heads from there instead of
+// walking vals -> pts -> val in global memory at every step.  A field's rows are contiguous
+// in the sorted list (each row starts where the previous one ends), so the workgroup's
+// points are one range of it.
 constexpr int kExactInjectThreads = 64;
+constexpr int kInjectStage = 1024;
+
+struct InjectRec {
+    int x;          // base column
+    uint32_t p;     // point index
+    double wx, wy;  // position within the base cell, x and y
+    double vc;      // value minus the field's offset
+};
+
+// head of sub-stream q at entry e: target x, point, corner weight, value (x = INT_MAX when
+// the stream is exhausted); STAGED reads the entry's record from LDS.  q may differ between
+// the lanes: its corner (q & 1, (q >> 1) & 1, q >> 2) selects the weight's factors.
+template <bool STAGED>
+__device__ __forceinline__ void inject_head(const FieldDev &F, int q, uint32_t e, uint32_t end,
+                                            const InjectRec *rec, uint32_t e0, const uint32_t *__restrict__ vals,
+                                            const double *__restrict__ pts, const double *__restrict__ val,
+                                            double off, int &x, uint32_t &p, double &w, double &v)
+{
+    if (e >= end) {
+        x = INT_MAX;
+        p = 0;
+        w = v = 0.0;
+        return;
+    }
+    PointPos pp;
+    if (STAGED) {
+        const InjectRec &r = rec[e - e0];
+        pp.i[0] = r.x;
+        pp.i[1] = pp.i[2] = 0;
+        pp.w[0] = r.wx;
+        pp.w[1] = r.wy;
+        pp.w[2] = 0.0;
+        p = r.p;
+        v = r.vc;
+    } else {
+        p = vals[e];
+        point_pos(F, pts + (int64_t)p * F.dim, pp);
+        v = val[p] - off;
+    }
+    x = (int)pp.i[0] + (q & 1);
+    w = corner_weight(F.dim, pp, q & 1, (q >> 1) & 1, q >> 2);
+}
+
+// the merge of one line's NS sub-streams (4 in 1D and 2D, 8 in 3D).  Each step takes the
+// smallest head and reads the next entry of that stream only: one read per step whichever
+// stream each lane advances.  A cell's sums start at 0.0 (ZEROED) or at the grids' values,
+// and are stored when the merge moves past the cell.
+template <int NS, bool STAGED, bool ZEROED>
+__device__ __forceinline__ void inject_merge(const FieldDev &F, uint32_t (&pos)[8], const uint32_t (&end)[8],
+                                             const InjectRec *rec, uint32_t e0, const uint32_t *__restrict__ vals,
+                                             const double *__restrict__ pts, const double *__restrict__ val,
+                                             double off, double *__restrict__ vg, double *__restrict__ wg,
+                                             int64_t base)
+{
+    // hr: entries left in the stream after its head (loop-carried, like the heads, so that
+    // taking the advanced stream's values stays a chain of selects, not an indexed array)
+    uint32_t hp[NS], hr[NS];
+    int hx[NS];
+    double hw[NS], hv[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+        inject_head<STAGED>(F, q, pos[q], end[q], rec, e0, vals, pts, val, off, hx[q], hp[q], hw[q], hv[q]);
+        hr[q] = pos[q] < end[q] ? end[q] - pos[q] - 1 : 0;
+    }
+    int cx = INT_MAX;
+    double av = 0.0, aw = 0.0;
+    while (true) {
+        int best = 0;
+#pragma unroll
+        for (int q = 1; q < NS; ++q)
+            if (hx[q] < hx[best] || (hx[q] == hx[best] && hp[q] < hp[best])) best = q;
+        int bx = hx[0];
+        double bw = hw[0], bv = hv[0];
+        uint32_t e = pos[0] + 1, r = hr[0];
+#pragma unroll
+        for (int q = 1; q < NS; ++q)
+            if (q == best) {
+                bx = hx[q];
+                bw = hw[q];
+                bv = hv[q];
+                e = pos[q] + 1;
+                r = hr[q];
+            }
+        // every stream exhausted (at once for a line without points): store the last cell
+        if (bx == INT_MAX) {
+            if (cx != INT_MAX) {
+                vg[base + cx] = av;
+                wg[base + cx] = aw;
+            }
+            break;
+        }
+        int nx;
+        uint32_t np;
+        double nw, nv;
+        inject_head<STAGED>(F, best, e, r ? e + 1 : e, rec, e0, vals, pts, val, off, nx, np, nw, nv);
+#pragma unroll
+        for (int q = 0; q < NS; ++q)
+            if (q == best) {
+                pos[q] = e;
+                hr[q] = r ? r - 1 : 0;
+                hx[q] = nx;
+                hp[q] = np;
+                hw[q] = nw;
+                hv[q] = nv;
+            }
+        if (bx != cx) {
+            if (cx != INT_MAX) {
+                vg[base + cx] = av;
+                wg[base + cx] = aw;
+            }
+            cx = bx;
+            av = ZEROED ? 0.0 : vg[base + cx];
+            aw = ZEROED ? 0.0 : wg[base + cx];
+        }
+        av += bw * bv;
+        aw += bw;
+    }
+}
 
 __global__ __launch_bounds__(kExactInjectThreads) void k_exact_inject(
     const FieldDev *__restrict__ fields, const int64_t *__restrict__ xline_prefix, int nfields, int64_t nxlines,
     const uint32_t *__restrict__ vals, const uint32_t *__restrict__ row_start,
     const uint32_t *__restrict__ row_count, const double *__restrict__ pts, const double *__restrict__ val,
-    const double *__restrict__ offsets, double *__restrict__ vg, double *__restrict__ wg)
+    const double *__restrict__ offsets, double *__restrict__ vg, double *__restrict__ wg, int zeroed)
 {
-    const int64_t gl = (int64_t)blockIdx.x * kExactInjectThreads + threadIdx.x;
-    if (gl >= nxlines) return;
-    const int f = find_seg(xline_prefix, nfields, gl);
+    __shared__ InjectRec rec[kInjectStage];
+    const int64_t g0 = (int64_t)blockIdx.x * kExactInjectThreads;
+    const int64_t glast = min(g0 + kExactInjectThreads - 1, nxlines - 1);
+    const int64_t gl = g0 + threadIdx.x;
+    const bool live = gl < nxlines;
+    const int f = find_seg(xline_prefix, nfields, live ? gl : glast);
     const FieldDev &F = fields[f];
-    const int64_t line = gl - F.xline_off;
+    const int64_t line = (live ? gl : glast) - F.xline_off;
     const int64_t base = F.cell_off + line * F.size[0];
     const double off = offsets[f];
-    uint32_t pos[8], end[8], hp[8];
-    int hx[8];
-    double hw[8];
+    uint32_t pos[8], end[8];
     {
         uint32_t rs[8], rc[8];
-        line_streams(F, line, true, row_start, row_count, rs, rc);
+        line_streams(F, line, live, row_start, row_count, rs, rc);
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             pos[q] = rs[q];
             end[q] = rs[q] + rc[q];
         }
     }
-    // head of sub-stream q: target x, point, corner weight (INT_MAX when exhausted)
-    auto head = [&](int q, uint32_t e, int &x, uint32_t &p, double &w) {
-        if (e >= end[q]) {
-            x = INT_MAX;
-            p = 0;
-            w = 0.0;
-            return;
-        }
-        p = vals[e];
-        PointPos pp;
-        point_pos(F, pts + (int64_t)p * F.dim, pp);
-        x = (int)pp.i[0] + (q & 1);
-        w = corner_weight(F.dim, pp, q & 1, (q >> 1) & 1, q >> 2);
-    };
+    // stage the workgroup's points when its lines share one 1D/2D field (then F is the same
+    // for every lane and both conditions are uniform) and every stream lies in their range
+    bool staged = false;
+    uint32_t e0 = 0;
+    if (find_seg(xline_prefix, nfields, g0) == find_seg(xline_prefix, nfields, glast) && F.dim <= 2) {
+        const int64_t lo = (F.dim == 2 && g0 > F.xline_off) ? g0 - 1 : g0;
+        e0 = row_start[lo];
+        const uint32_t e1 = row_start[glast] + row_count[glast];
+        bool ok = e0 <= e1 && e1 - e0 <= (uint32_t)kInjectStage;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) head(q, pos[q], hx[q], hp[q], hw[q]);
-    while (true) {
-        int best = 0;
-#pragma unroll
-        for (int q = 1; q < 8; ++q)
-            if (hx[q] < hx[best] || (hx[q] == hx[best] && hp[q] < hp[best])) best = q;
-        int bx = INT_MAX;
-        uint32_t bp = 0;
-        double bw = 0.0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-            if (q == best) {
-                bx = hx[q];
-                bp = hp[q];
-                bw = hw[q];
-                ++pos[q];
-                head(q, pos[q], hx[q], hp[q], hw[q]);
+        for (int q = 0; q < 8; ++q) ok = ok && (pos[q] == end[q] || (pos[q] >= e0 && end[q] <= e1));
+        staged = __all(ok);
+        if (staged) {
+            for (uint32_t e = e0 + threadIdx.x; e < e1; e += kExactInjectThreads) {
+                const uint32_t p = vals[e];
+                PointPos pp;
+                point_pos(F, pts + (int64_t)p * F.dim, pp);
+                InjectRec &r = rec[e - e0];
+                r.x = (int)pp.i[0];
+                r.p = p;
+                r.wx = pp.w[0];
+                r.wy = pp.w[1];
+                r.vc = val[p] - off;
             }
-        if (bx == INT_MAX) break;
-        const double vc = val[bp] - off;
-        vg[base + bx] += bw * vc;
-        wg[base + bx] += bw;
+            __syncthreads();
+        }
+    }
+    if (!live) return;
+    // staged lines are 1D or 2D (4 streams)
+    if (staged) {
+        if (zeroed)
+            inject_merge<4, true, true>(F, pos, end, rec, e0, vals, pts, val, off, vg, wg, base);
+        else
+            inject_merge<4, true, false>(F, pos, end, rec, e0, vals, pts, val, off, vg, wg, base);
+    } else if (F.dim <= 2) {
+        if (zeroed)
+            inject_merge<4, false, true>(F, pos, end, rec, e0, vals, pts, val, off, vg, wg, base);
+        else
+            inject_merge<4, false, false>(F, pos, end, rec, e0, vals, pts, val, off, vg, wg, base);
+    } else {
+        if (zeroed)
+            inject_merge<8, false, true>(F, pos, end, rec, e0, vals, pts, val, off, vg, wg, base);
+        else
+            inject_merge<8, false, false>(F, pos, end, rec, e0, vals, pts, val, off, vg, wg, base);
     }
 }
 
@@ -2064,12 +2201,12 @@ int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int 
 int launch_exact_inject(const FieldDev *d_fields, const int64_t *d_xline_prefix, int nfields, int64_t nxlines,
                         const uint32_t *d_vals, const uint32_t *d_row_start, const uint32_t *d_row_count,
                         const double *d_pts, const double *d_val, const double *d_offsets, double *d_vg,
-                        double *d_wg, hipStream_t s)
+                        double *d_wg, bool zeroed, hipStream_t s)
 {
     if (nxlines <= 0) return FB_OK;
     hipLaunchKernelGGL(k_exact_inject, dim3((unsigned)((nxlines + kExactInjectThreads - 1) / kExactInjectThreads)),
                        dim3(kExactInjectThreads), 0, s, d_fields, d_xline_prefix, nfields, nxlines, d_vals,
-                       d_row_start, d_row_count, d_pts, d_val, d_offsets, d_vg, d_wg);
+                       d_row_start, d_row_count, d_pts, d_val, d_offsets, d_vg, d_wg, zeroed ? 1 : 0);
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }

@@ -94,3 +94,71 @@ def test_quantised_interpolation_sequence_matches_oracle(orc):
     ok = ~np.isnan(rwg)
     assert np.array_equal(vg[ok].view(np.uint64), rvg[ok].view(np.uint64))
     assert np.array_equal(got.astype(np.float32)[ok].view(np.uint32), ref[ok].view(np.uint32))
+
+
+def _row_points(rng, counts, width, step, x0, crowd_rows=()):
+    """2D points whose base rows hold counts[r] points each (rows listed in crowd_rows put
+    their points into 3 cells), in random point order."""
+    ys, xs = [], []
+    for r, c in enumerate(counts):
+        if c == 0:
+            continue
+        if r in crowd_rows:
+            x = rng.integers(0, 3, c) * 7.0 + rng.uniform(0.01, 0.99, c)
+        else:
+            x = rng.uniform(0.0, width - 1.0, c)
+        xs.append(x)
+        ys.append(r + rng.uniform(0.01, 0.99, c))
+    g = np.stack([np.concatenate(xs), np.concatenate(ys)], axis=1)
+    g = g[rng.permutation(len(g))]
+    return x0 + g * step
+
+
+@pytest.mark.parametrize('dim', [1, 2])
+def test_inject_staged_and_global_workgroups(orc, dim):
+    """The FP64 injection stages a workgroup's points in LDS when its 64 lines lie in one 1D or
+    2D field and their rows (with, in 2D, the row below the first line) hold at most 1024
+    points; otherwise it merges from global memory.  Workgroups of both kinds, at the boundary
+    (1024 and 1025 points) and with many points per cell, give the oracle's grids bit for bit:
+    added into the caller's grids, and through the FP64 pipeline (cleared grids)."""
+    from fastbarnes import interpolation as ip
+    rng = np.random.default_rng(7 + dim)
+    if dim == 2:
+        width, height = 50, 256
+        counts = np.zeros(height, dtype=np.int64)
+        counts[:63] = rng.multinomial(1000, np.full(63, 1 / 63))    # lines 0-63: rows 0-63, 1024
+        counts[63] = 24
+        counts[64:127] = rng.multinomial(1000, np.full(63, 1 / 63))  # lines 64-127: rows 63-127, 1025
+        counts[127] = 1
+        counts[128:192] = rng.multinomial(200, np.full(64, 1 / 64))  # lines 128-191: 201, crowded
+        counts[200:211] = 200                                       # lines 192-255: 2200
+        crowd = set(range(130, 140)) | set(range(200, 205))
+        step = np.array([0.5, 0.25])
+        x0 = np.array([1.0, -2.0])
+        size = (width, height)
+        cases = [_row_points(rng, counts, width, step, x0, crowd)]
+    else:
+        step = np.array([0.5])
+        x0 = np.array([3.0])
+        size = (3000,)
+        cases = []
+        for n in (1024, 1025):
+            g = rng.uniform(0.0, 2999.0, n)
+            g[:200] = np.floor(g[:200] / 500.0) + rng.uniform(0.01, 0.99, 200)   # 6 crowded cells
+            cases.append((x0 + g * step)[:, None])
+    for pts in cases:
+        n = len(pts)
+        val = rng.normal(0.0, 3.0, n)
+        shape = tuple(size[::-1])
+        vg0 = rng.normal(0.0, 1.0, shape)
+        wg0 = rng.uniform(0.0, 2.0, shape)
+        vg, wg = vg0.copy(), wg0.copy()
+        (ip._inject_data_2d if dim == 2 else ip._inject_data_1d)(vg, wg, pts, val, x0, step, size)
+        rv, rw = vg0.copy(), wg0.copy()
+        orc.inject_raw(rv, rw, pts, val, x0, step, size)
+        assert np.array_equal(vg.view(np.uint64), rv.view(np.uint64)), n
+        assert np.array_equal(wg.view(np.uint64), rw.view(np.uint64)), n
+        out = ip.barnes(pts, val, 1.0, x0, step, size, method='optimized_convolution', num_iter=4,
+                        max_dist=3.5, precision='fp64')
+        ref = orc.interpolate_convol(pts, val, 1.0, x0, step, size, num_iter=4, max_dist=3.5)
+        assert_bitwise(out, ref)
