@@ -498,6 +498,7 @@ struct fb_plan {
     uint32_t *d_line_count = nullptr, *d_line_start = nullptr, *d_scan_tmp = nullptr;
     uint32_t *d_skeys = nullptr, *d_big_list = nullptr, *d_big_count = nullptr;   // row buckets
     PointRec *d_rec = nullptr;                  // FP32C: each point's position for the injection
+    InjectRec *d_irec = nullptr;                // FP64: each sorted entry's injection record
     double *d_vg = nullptr, *d_wg = nullptr, *d_vs = nullptr, *d_ws = nullptr;
     float *d_v = nullptr, *d_w = nullptr;
     PassWork *d_work[3] = {nullptr, nullptr, nullptr};
@@ -992,11 +993,13 @@ int plan_create(fb_plan **out, const fb_field *fields, int nfields, int precisio
     TRY(P->alloc(&P->d_scan_tmp, scan_temp_elems(P->nxlines)));
     if (!grids) {
         // the sort and injection stages only (fb_inject): the caller's grids take the sums
+        TRY(P->alloc(&P->d_irec, P->npoints));
     } else if (!fast) {
         TRY(P->alloc(&P->d_vg, P->ncells));
         TRY(P->alloc(&P->d_wg, P->ncells));
         TRY(P->alloc(&P->d_vs, P->ncells));
         TRY(P->alloc(&P->d_ws, P->ncells));
+        TRY(P->alloc(&P->d_irec, P->npoints));
     } else {
         TRY(P->alloc(&P->d_v, P->ncells));
         TRY(P->alloc(&P->d_w, P->ncells));
@@ -1076,8 +1079,11 @@ int plan_run(fb_plan *P, const double *pts, const double *val, float *out, float
     if (P->precision == FB_PREC_FP64) {
         if ((rc = launch_zero(P->d_vg, P->ncells * 8, s))) return rc;
         if ((rc = launch_zero(P->d_wg, P->ncells * 8, s))) return rc;
-        if ((rc = launch_exact_inject(P->d_fields, P->d_xline_prefix, P->nfields, P->nxlines, vs, P->d_line_start,
-                                      P->d_line_count, pts, val, P->d_offsets, P->d_vg, P->d_wg, true, s)))
+        if ((rc = launch_inject_records(P->d_fields, P->d_pt_prefix, P->nfields, P->npoints,
+                                        P->max_n <= kLocalSortMax, P->nxlines, vs, P->d_line_start,
+                                        P->d_line_count, pts, val, P->d_offsets, P->d_irec, s)) ||
+            (rc = launch_exact_inject(P->d_fields, P->d_xline_prefix, P->nfields, P->nxlines, P->d_irec,
+                                      P->d_line_start, P->d_line_count, P->d_vg, P->d_wg, P->dim, true, s)))
             return rc;
         stage_mark(P, s, "inject");
         static const char *ln[3] = {"lines_x", "lines_y", "lines_z"};
@@ -1826,8 +1832,12 @@ int fb_inject(int dim, int64_t n, const double *pts, const double *val, const do
         rc = fail(FB_ERR_HIP, "clearing the offset failed");
     }
     if (!rc)
-        rc = launch_exact_inject(P->d_fields, P->d_xline_prefix, 1, P->nxlines, vs, P->d_line_start,
-                                 P->d_line_count, d_pts, d_val, P->d_offsets, d_vg, d_wg, false, s);
+        rc = launch_inject_records(P->d_fields, P->d_pt_prefix, 1, P->npoints, P->max_n <= kLocalSortMax,
+                                   P->nxlines, vs, P->d_line_start, P->d_line_count, d_pts, d_val, P->d_offsets,
+                                   P->d_irec, s);
+    if (!rc)
+        rc = launch_exact_inject(P->d_fields, P->d_xline_prefix, 1, P->nxlines, P->d_irec, P->d_line_start,
+                                 P->d_line_count, d_vg, d_wg, P->dim, false, s);
     if (!rc && !on_dev) {
         if (hipMemcpyAsync(vg, d_vg, cells * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipMemcpyAsync(wg, d_wg, cells * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) {

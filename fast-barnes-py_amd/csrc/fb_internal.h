@@ -153,6 +153,14 @@ inline int exact_rows_slots(int64_t slots)
     return s;
 }
 
+// FP64 injection: one sorted entry's record (k_inject_records), in list order.
+struct InjectRec {
+    int x;              // base column
+    uint32_t p;         // point index
+    double wx, wy, wz;  // position within the base cell
+    double vc;          // value minus the field's offset
+};
+
 // Contribution record of the sorted injection list.
 struct Contrib {
     uint32_t key;          // global cell index (kInvalidKey when outside)

@@ -432,6 +432,33 @@ __device__ __forceinline__ void line_streams(const FieldDev &F, int64_t line, bo
 // ---------------------------------------------------------------------------
 // FP64 path, reference operation order
 
+// Each sorted entry's injection record, in list order (the merge then reads one record per
+// step): base column, point, position within the base cell and value minus the offset, as
+// point_pos forms them.  The per-field sort (`local`) writes a field's in-grid points from its
+// first point index on and leaves the slots after them unused; the row-bucket sort packs all
+// fields' entries by row.  Slots past the list's last entry are skipped.
+__global__ __launch_bounds__(kBlock) void k_inject_records(
+    const FieldDev *__restrict__ fields, const int64_t *__restrict__ pt_prefix, int nfields, int64_t npoints,
+    int local, int64_t nxlines, const uint32_t *__restrict__ vals, const uint32_t *__restrict__ row_start,
+    const uint32_t *__restrict__ row_count, const double *__restrict__ pts, const double *__restrict__ val,
+    const double *__restrict__ offsets, InjectRec *__restrict__ irec)
+{
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= npoints) return;
+    int64_t last = nxlines - 1;   // the list's last row
+    if (local) {
+        const FieldDev &Fe = fields[find_seg(pt_prefix, nfields, e)];
+        last = Fe.xline_off + Fe.size[1] * Fe.size[2] - 1;   // the last row of the slot's field
+    }
+    if (e >= (int64_t)row_start[last] + row_count[last]) return;
+    const uint32_t p = vals[e];
+    const int f = find_seg(pt_prefix, nfields, p);
+    const FieldDev &F = fields[f];
+    PointPos pp;
+    point_pos(F, pts + (int64_t)p * F.dim, pp);
+    irec[e] = InjectRec{(int)pp.i[0], p, pp.w[0], pp.w[1], pp.w[2], val[p] - offsets[f]};
+}
+
 // One thread per x-line: merges the line's sub-streams by (x, point index) and adds
 // w*val_c and w to the FP64 grids in that order, which is the per-cell order of the
 // reference's point loop (each cell gets its contributions in increasing point index).
@@ -440,31 +467,23 @@ __device__ __forceinline__ void line_streams(const FieldDev &F, int64_t line, bo
 // else at the grids' values.  One wave per workgroup, as k_exact_lines: a field has only as
 // many threads as x lines.
 //
-// The merge is a chain of dependent steps.  When a workgroup's lines lie in one 1D or 2D
-// field and their rows (each line's own and, in 2D, the row below) hold at most
-// kInjectStage points, the wave first stages those points' records in LDS, all of their
-// loads and divisions in parallel, and the merge reads its heads from there instead of
-// walking vals -> pts -> val in global memory at every step.  A field's rows are contiguous
-// in the sorted list (each row starts where the previous one ends), so the workgroup's
-// points are one range of it.
+// The merge is a chain of dependent steps, one entry record read per step.  When a
+// workgroup's lines lie in one 1D or 2D field and their rows (each line's own and, in 2D,
+// the row below) hold at most stage_cap points, the wave first copies those records to LDS
+// (dynamic, stage_cap records: kInjectStage for plans of 1D and 2D fields, none for 3D) and
+// the merge reads them there.  A field's rows are contiguous in the sorted list (each
+// row starts where the previous one ends), so the workgroup's records are one range of it.
 constexpr int kExactInjectThreads = 64;
 constexpr int kInjectStage = 1024;
 
-struct InjectRec {
-    int x;          // base column
-    uint32_t p;     // point index
-    double wx, wy;  // position within the base cell, x and y
-    double vc;      // value minus the field's offset
-};
-
 // head of sub-stream q at entry e: target x, point, corner weight, value (x = INT_MAX when
-// the stream is exhausted); STAGED reads the entry's record from LDS.  q may differ between
-// the lanes: its corner (q & 1, (q >> 1) & 1, q >> 2) selects the weight's factors.
+// the stream is exhausted); STAGED reads the record from LDS.  q may differ between the
+// lanes: its corner (q & 1, (q >> 1) & 1, q >> 2) selects the weight's factors.
 template <bool STAGED>
 __device__ __forceinline__ void inject_head(const FieldDev &F, int q, uint32_t e, uint32_t end,
-                                            const InjectRec *rec, uint32_t e0, const uint32_t *__restrict__ vals,
-                                            const double *__restrict__ pts, const double *__restrict__ val,
-                                            double off, int &x, uint32_t &p, double &w, double &v)
+                                            const InjectRec *rec, uint32_t e0,
+                                            const InjectRec *__restrict__ irec, int &x, uint32_t &p,
+                                            double &w, double &v)
 {
     if (e >= end) {
         x = INT_MAX;
@@ -472,22 +491,16 @@ __device__ __forceinline__ void inject_head(const FieldDev &F, int q, uint32_t e
         w = v = 0.0;
         return;
     }
+    const InjectRec &r = STAGED ? rec[e - e0] : irec[e];
     PointPos pp;
-    if (STAGED) {
-        const InjectRec &r = rec[e - e0];
-        pp.i[0] = r.x;
-        pp.i[1] = pp.i[2] = 0;
-        pp.w[0] = r.wx;
-        pp.w[1] = r.wy;
-        pp.w[2] = 0.0;
-        p = r.p;
-        v = r.vc;
-    } else {
-        p = vals[e];
-        point_pos(F, pts + (int64_t)p * F.dim, pp);
-        v = val[p] - off;
-    }
-    x = (int)pp.i[0] + (q & 1);
+    pp.i[0] = r.x;
+    pp.i[1] = pp.i[2] = 0;
+    pp.w[0] = r.wx;
+    pp.w[1] = r.wy;
+    pp.w[2] = r.wz;
+    p = r.p;
+    v = r.vc;
+    x = r.x + (q & 1);
     w = corner_weight(F.dim, pp, q & 1, (q >> 1) & 1, q >> 2);
 }
 
@@ -497,10 +510,9 @@ __device__ __forceinline__ void inject_head(const FieldDev &F, int q, uint32_t e
 // and are stored when the merge moves past the cell.
 template <int NS, bool STAGED, bool ZEROED>
 __device__ __forceinline__ void inject_merge(const FieldDev &F, uint32_t (&pos)[8], const uint32_t (&end)[8],
-                                             const InjectRec *rec, uint32_t e0, const uint32_t *__restrict__ vals,
-                                             const double *__restrict__ pts, const double *__restrict__ val,
-                                             double off, double *__restrict__ vg, double *__restrict__ wg,
-                                             int64_t base)
+                                             const InjectRec *rec, uint32_t e0,
+                                             const InjectRec *__restrict__ irec, double *__restrict__ vg,
+                                             double *__restrict__ wg, int64_t base)
 {
     // hr: entries left in the stream after its head (loop-carried, like the heads, so that
     // taking the advanced stream's values stays a chain of selects, not an indexed array)
@@ -509,7 +521,7 @@ __device__ __forceinline__ void inject_merge(const FieldDev &F, uint32_t (&pos)[
     double hw[NS], hv[NS];
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
-        inject_head<STAGED>(F, q, pos[q], end[q], rec, e0, vals, pts, val, off, hx[q], hp[q], hw[q], hv[q]);
+        inject_head<STAGED>(F, q, pos[q], end[q], rec, e0, irec, hx[q], hp[q], hw[q], hv[q]);
         hr[q] = pos[q] < end[q] ? end[q] - pos[q] - 1 : 0;
     }
     int cx = INT_MAX;
@@ -542,7 +554,7 @@ __device__ __forceinline__ void inject_merge(const FieldDev &F, uint32_t (&pos)[
         int nx;
         uint32_t np;
         double nw, nv;
-        inject_head<STAGED>(F, best, e, r ? e + 1 : e, rec, e0, vals, pts, val, off, nx, np, nw, nv);
+        inject_head<STAGED>(F, best, e, r ? e + 1 : e, rec, e0, irec, nx, np, nw, nv);
 #pragma unroll
         for (int q = 0; q < NS; ++q)
             if (q == best) {
@@ -569,11 +581,11 @@ __device__ __forceinline__ void inject_merge(const FieldDev &F, uint32_t (&pos)[
 
 __global__ __launch_bounds__(kExactInjectThreads) void k_exact_inject(
     const FieldDev *__restrict__ fields, const int64_t *__restrict__ xline_prefix, int nfields, int64_t nxlines,
-    const uint32_t *__restrict__ vals, const uint32_t *__restrict__ row_start,
-    const uint32_t *__restrict__ row_count, const double *__restrict__ pts, const double *__restrict__ val,
-    const double *__restrict__ offsets, double *__restrict__ vg, double *__restrict__ wg, int zeroed)
+    const InjectRec *__restrict__ irec, const uint32_t *__restrict__ row_start,
+    const uint32_t *__restrict__ row_count, double *__restrict__ vg, double *__restrict__ wg, int zeroed,
+    int stage_cap)
 {
-    __shared__ InjectRec rec[kInjectStage];
+    extern __shared__ InjectRec rec[];
     const int64_t g0 = (int64_t)blockIdx.x * kExactInjectThreads;
     const int64_t glast = min(g0 + kExactInjectThreads - 1, nxlines - 1);
     const int64_t gl = g0 + threadIdx.x;
@@ -582,7 +594,6 @@ __global__ __launch_bounds__(kExactInjectThreads) void k_exact_inject(
     const FieldDev &F = fields[f];
     const int64_t line = (live ? gl : glast) - F.xline_off;
     const int64_t base = F.cell_off + line * F.size[0];
-    const double off = offsets[f];
     uint32_t pos[8], end[8];
     {
         uint32_t rs[8], rc[8];
@@ -593,7 +604,7 @@ __global__ __launch_bounds__(kExactInjectThreads) void k_exact_inject(
             end[q] = rs[q] + rc[q];
         }
     }
-    // stage the workgroup's points when its lines share one 1D/2D field (then F is the same
+    // stage the workgroup's records when its lines share one 1D/2D field (then F is the same
     // for every lane and both conditions are uniform) and every stream lies in their range
     bool staged = false;
     uint32_t e0 = 0;
@@ -601,22 +612,12 @@ __global__ __launch_bounds__(kExactInjectThreads) void k_exact_inject(
         const int64_t lo = (F.dim == 2 && g0 > F.xline_off) ? g0 - 1 : g0;
         e0 = row_start[lo];
         const uint32_t e1 = row_start[glast] + row_count[glast];
-        bool ok = e0 <= e1 && e1 - e0 <= (uint32_t)kInjectStage;
+        bool ok = e0 <= e1 && e1 - e0 <= (uint32_t)stage_cap;
 #pragma unroll
         for (int q = 0; q < 8; ++q) ok = ok && (pos[q] == end[q] || (pos[q] >= e0 && end[q] <= e1));
         staged = __all(ok);
         if (staged) {
-            for (uint32_t e = e0 + threadIdx.x; e < e1; e += kExactInjectThreads) {
-                const uint32_t p = vals[e];
-                PointPos pp;
-                point_pos(F, pts + (int64_t)p * F.dim, pp);
-                InjectRec &r = rec[e - e0];
-                r.x = (int)pp.i[0];
-                r.p = p;
-                r.wx = pp.w[0];
-                r.wy = pp.w[1];
-                r.vc = val[p] - off;
-            }
+            for (uint32_t e = e0 + threadIdx.x; e < e1; e += kExactInjectThreads) rec[e - e0] = irec[e];
             __syncthreads();
         }
     }
@@ -624,19 +625,19 @@ __global__ __launch_bounds__(kExactInjectThreads) void k_exact_inject(
     // staged lines are 1D or 2D (4 streams)
     if (staged) {
         if (zeroed)
-            inject_merge<4, true, true>(F, pos, end, rec, e0, vals, pts, val, off, vg, wg, base);
+            inject_merge<4, true, true>(F, pos, end, rec, e0, irec, vg, wg, base);
         else
-            inject_merge<4, true, false>(F, pos, end, rec, e0, vals, pts, val, off, vg, wg, base);
+            inject_merge<4, true, false>(F, pos, end, rec, e0, irec, vg, wg, base);
     } else if (F.dim <= 2) {
         if (zeroed)
-            inject_merge<4, false, true>(F, pos, end, rec, e0, vals, pts, val, off, vg, wg, base);
+            inject_merge<4, false, true>(F, pos, end, rec, e0, irec, vg, wg, base);
         else
-            inject_merge<4, false, false>(F, pos, end, rec, e0, vals, pts, val, off, vg, wg, base);
+            inject_merge<4, false, false>(F, pos, end, rec, e0, irec, vg, wg, base);
     } else {
         if (zeroed)
-            inject_merge<8, false, true>(F, pos, end, rec, e0, vals, pts, val, off, vg, wg, base);
+            inject_merge<8, false, true>(F, pos, end, rec, e0, irec, vg, wg, base);
         else
-            inject_merge<8, false, false>(F, pos, end, rec, e0, vals, pts, val, off, vg, wg, base);
+            inject_merge<8, false, false>(F, pos, end, rec, e0, irec, vg, wg, base);
     }
 }
 
@@ -2198,15 +2199,36 @@ int launch_point_keys(const FieldDev *d_fields, const int64_t *d_pt_prefix, int 
     return FB_OK;
 }
 
+int launch_inject_records(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
+                          bool local, int64_t nxlines, const uint32_t *d_vals, const uint32_t *d_row_start,
+                          const uint32_t *d_row_count, const double *d_pts, const double *d_val,
+                          const double *d_offsets, InjectRec *d_irec, hipStream_t s)
+{
+    if (npoints <= 0 || nxlines <= 0) return FB_OK;
+    if (!d_irec) {
+        set_error("FP64 injection: the plan has no record array");
+        return FB_ERR_INVALID_ARGUMENT;
+    }
+    hipLaunchKernelGGL(k_inject_records, dim3((unsigned)((npoints + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       d_fields, d_pt_prefix, nfields, npoints, local ? 1 : 0, nxlines, d_vals, d_row_start,
+                       d_row_count, d_pts, d_val, d_offsets, d_irec);
+    FB_HIP_CHECK(hipGetLastError());
+    return FB_OK;
+}
+
 int launch_exact_inject(const FieldDev *d_fields, const int64_t *d_xline_prefix, int nfields, int64_t nxlines,
-                        const uint32_t *d_vals, const uint32_t *d_row_start, const uint32_t *d_row_count,
-                        const double *d_pts, const double *d_val, const double *d_offsets, double *d_vg,
-                        double *d_wg, bool zeroed, hipStream_t s)
+                        const InjectRec *d_irec, const uint32_t *d_row_start, const uint32_t *d_row_count,
+                        double *d_vg, double *d_wg, int dim, bool zeroed, hipStream_t s)
 {
     if (nxlines <= 0) return FB_OK;
+    if (!d_irec) {
+        set_error("FP64 injection: the plan has no record array");
+        return FB_ERR_INVALID_ARGUMENT;
+    }
+    const int cap = dim <= 2 ? kInjectStage : 0;   // 3D lines never stage
     hipLaunchKernelGGL(k_exact_inject, dim3((unsigned)((nxlines + kExactInjectThreads - 1) / kExactInjectThreads)),
-                       dim3(kExactInjectThreads), 0, s, d_fields, d_xline_prefix, nfields, nxlines, d_vals,
-                       d_row_start, d_row_count, d_pts, d_val, d_offsets, d_vg, d_wg, zeroed ? 1 : 0);
+                       dim3(kExactInjectThreads), (size_t)cap * sizeof(InjectRec), s, d_fields, d_xline_prefix,
+                       nfields, nxlines, d_irec, d_row_start, d_row_count, d_vg, d_wg, zeroed ? 1 : 0, cap);
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }

@@ -49,10 +49,14 @@ int row_bucket_sort(const uint32_t *d_keys, const uint32_t *d_slot, const uint32
                     uint32_t *d_tvals, hipStream_t stream);
 
 // fb_kernels.hip -- FP64 path with the reference's operation order
+// Each sorted entry's FP64 injection record (local: the per-field sort's layout).
+int launch_inject_records(const FieldDev *d_fields, const int64_t *d_pt_prefix, int nfields, int64_t npoints,
+                          bool local, int64_t nxlines, const uint32_t *d_vals, const uint32_t *d_row_start,
+                          const uint32_t *d_row_count, const double *d_pts, const double *d_val,
+                          const double *d_offsets, InjectRec *d_irec, hipStream_t stream);
 int launch_exact_inject(const FieldDev *d_fields, const int64_t *d_xline_prefix, int nfields, int64_t nxlines,
-                        const uint32_t *d_vals, const uint32_t *d_row_start, const uint32_t *d_row_count,
-                        const double *d_pts, const double *d_val, const double *d_offsets, double *d_vg,
-                        double *d_wg, bool zeroed, hipStream_t stream);
+                        const InjectRec *d_irec, const uint32_t *d_row_start, const uint32_t *d_row_count,
+                        double *d_vg, double *d_wg, int dim, bool zeroed, hipStream_t stream);
 // One thread per (line, field component); lines of axis `axis` of every field.
 // line_prefix[f] = first (2*line) slot of field f; total = line_prefix[nfields].
 // rect_len < 0 selects 2*T[axis]+1 from the field.
