@@ -131,10 +131,15 @@ struct PointRec {
 // a kernel of rect terms (0 when a 64-slot workgroup's ring would not fit the LDS);
 // exact_rows_lds: the bytes of a workgroup of `slots` slots.
 constexpr int kRowTile = 32;
-__host__ __device__ inline int64_t exact_rows_slot_stride(int ring) { return (int64_t)ring * kRowTile + 1; }
-__host__ __device__ inline int64_t exact_rows_lds(int ring, int slots)
+// mir: the ring's first kRowTile + 1 positions repeated past its end, so that a chunk's reads
+// never wrap (slot stride odd either way)
+__host__ __device__ inline int64_t exact_rows_slot_stride(int ring, bool mir = false)
 {
-    return (slots * exact_rows_slot_stride(ring) + slots * (kRowTile + 1)) * (int64_t)sizeof(double);
+    return (int64_t)ring * kRowTile + (mir ? kRowTile + 1 : 1);
+}
+__host__ __device__ inline int64_t exact_rows_lds(int ring, int slots, bool mir = false)
+{
+    return (slots * exact_rows_slot_stride(ring, mir) + slots * (kRowTile + 1)) * (int64_t)sizeof(double);
 }
 inline int exact_rows_ring(int64_t rect)
 {

@@ -778,7 +778,7 @@ __global__ __launch_bounds__(kExactLineThreads) void k_exact_lines(const FieldDe
 // the next tile loads.  Slot stride in LDS: ring*C + 1 doubles (odd: the lanes' reads of one
 // column fall on different banks).
 
-template <bool TAIL, int S>
+template <bool TAIL, int S, bool MIR>
 __device__ __forceinline__ void exact_rows(const FieldDev &F, int64_t s0, int nsl, int rect_override, int ring,
                                            double *__restrict__ rl, double *__restrict__ vg,
                                            double *__restrict__ wg, double *__restrict__ vs,
@@ -792,7 +792,8 @@ __device__ __forceinline__ void exact_rows(const FieldDev &F, int64_t s0, int ns
     const int64_t h0 = (rect - 1) / 2, h0_1 = h0 + 1, h1 = rect - h0;
     const double alpha = F.alpha[0];
     const int mask = ring * C - 1;             // the ring holds ring*C columns: a power of two
-    const int64_t RS = exact_rows_slot_stride(ring);
+    const int64_t RS = exact_rows_slot_stride(ring, MIR);
+    const int rlen = ring * C;                 // the ring's positions; with MIR, [rlen, rlen + C] repeat [0, C]
     double *ot = rl + S * RS;                  // output tile, slot stride C + 1
     const int64_t NT = (L + C - 1) / C;
     // loader / storer mapping: instruction j takes slot 2j + (lane >> 5), column lane & 31; the
@@ -816,6 +817,10 @@ __device__ __forceinline__ void exact_rows(const FieldDev &F, int64_t s0, int ns
         const int p = (int)(t * C + cl) & mask;
 #pragma unroll
         for (int j = 0; j < NP; ++j) rl[(2 * j + wl) * RS + p] = pre[j];
+        if (MIR && p <= C) {
+#pragma unroll
+            for (int j = 0; j < NP; ++j) rl[(2 * j + wl) * RS + rlen + p] = pre[j];
+        }
     };
     auto in_at = [&](int64_t p) { return my[(int)p & mask]; };
     for (int it = 0; it < F.num_iter; ++it) {
@@ -842,10 +847,18 @@ __device__ __forceinline__ void exact_rows(const FieldDev &F, int64_t s0, int ns
                     // steps in the reference's order
                     double a[C + 1], b[C];
                     const int pa = (int)(k0 + h0), pb = (int)(k0 - h1);
+                    if (MIR) {   // positions (pa & mask) .. + C, read past the ring's end from the mirror
+                        const double *ma = my + (pa & mask), *mb = my + (pb & mask);
 #pragma unroll
-                    for (int j = 0; j <= C; ++j) a[j] = my[(pa + j) & mask];
+                        for (int j = 0; j <= C; ++j) a[j] = ma[j];
 #pragma unroll
-                    for (int j = 0; j < C; ++j) b[j] = my[(pb + j) & mask];
+                        for (int j = 0; j < C; ++j) b[j] = mb[j];
+                    } else {
+#pragma unroll
+                        for (int j = 0; j <= C; ++j) a[j] = my[(pa + j) & mask];
+#pragma unroll
+                        for (int j = 0; j < C; ++j) b[j] = my[(pb + j) & mask];
+                    }
 #pragma unroll
                     for (int j = 0; j < C; ++j) {
                         accu += (a[j] - b[j]);
@@ -927,7 +940,7 @@ __device__ __forceinline__ void exact_rows(const FieldDev &F, int64_t s0, int ns
 // f (slot 2l + c: x line l, field c = v, w of the field's 2*H*D slots).  S = 32 or 16 (lanes
 // past S load and store, but run no line) spreads the few lines of a small plan over more CUs,
 // each chunk then moving a half or a quarter of the bytes.
-template <int S>
+template <int S, bool MIR>
 __global__ __launch_bounds__(64) void k_exact_rows(const FieldDev *__restrict__ fields,
                                                    const int64_t *__restrict__ wg_prefix,
                                                    const int64_t *__restrict__ slot_prefix, int nfields,
@@ -940,8 +953,8 @@ __global__ __launch_bounds__(64) void k_exact_rows(const FieldDev *__restrict__ 
     const FieldDev &F = fields[f];
     const int64_t s0 = (blockIdx.x - wg_prefix[f]) * S;
     const int nsl = (int)min((int64_t)S, slot_prefix[f + 1] - slot_prefix[f] - s0);
-    if (F.tail) exact_rows<true, S>(F, s0, nsl, rect_override, ring, rl, vg, wg, vs, ws);
-    else exact_rows<false, S>(F, s0, nsl, rect_override, ring, rl, vg, wg, vs, ws);
+    if (F.tail) exact_rows<true, S, MIR>(F, s0, nsl, rect_override, ring, rl, vg, wg, vs, ws);
+    else exact_rows<false, S, MIR>(F, s0, nsl, rect_override, ring, rl, vg, wg, vs, ws);
 }
 
 // wg[wg < thr] = NaN, then (vg/wg + offset).astype(float32): interpolation.py:428-430, :367
@@ -2254,15 +2267,23 @@ int launch_exact_rows(const FieldDev *d_fields, const int64_t *d_wg_prefix, cons
         set_error("FP64 x lines: kernel of " + std::to_string(max_rect) + " terms too wide for the LDS ring");
         return FB_ERR_UNSUPPORTED;
     }
-    if (slots_per_wg == 16)
-        hipLaunchKernelGGL(k_exact_rows<16>, dim3((unsigned)nwg), dim3(64), (size_t)exact_rows_lds(ring, 16), s,
-                           d_fields, d_wg_prefix, d_slot_prefix, nfields, rect_override, ring, d_vg, d_wg, d_vs, d_ws);
-    else if (slots_per_wg == 32)
-        hipLaunchKernelGGL(k_exact_rows<32>, dim3((unsigned)nwg), dim3(64), (size_t)exact_rows_lds(ring, 32), s,
-                           d_fields, d_wg_prefix, d_slot_prefix, nfields, rect_override, ring, d_vg, d_wg, d_vs, d_ws);
-    else
-        hipLaunchKernelGGL(k_exact_rows<64>, dim3((unsigned)nwg), dim3(64), (size_t)exact_rows_lds(ring, 64), s,
-                           d_fields, d_wg_prefix, d_slot_prefix, nfields, rect_override, ring, d_vg, d_wg, d_vs, d_ws);
+    // the mirrored ring where its LDS fits (the plain ring's always does: exact_rows_ring)
+    const bool mir = exact_rows_lds(ring, slots_per_wg, true) <= 160 * 1024;
+    const size_t lds = (size_t)exact_rows_lds(ring, slots_per_wg, mir);
+#define FB_EXACT_ROWS(S_, M_)                                                                                       \
+    hipLaunchKernelGGL((k_exact_rows<S_, M_>), dim3((unsigned)nwg), dim3(64), lds, s, d_fields, d_wg_prefix,         \
+                       d_slot_prefix, nfields, rect_override, ring, d_vg, d_wg, d_vs, d_ws)
+    if (slots_per_wg == 16) {
+        if (mir) FB_EXACT_ROWS(16, true);
+        else FB_EXACT_ROWS(16, false);
+    } else if (slots_per_wg == 32) {
+        if (mir) FB_EXACT_ROWS(32, true);
+        else FB_EXACT_ROWS(32, false);
+    } else {
+        if (mir) FB_EXACT_ROWS(64, true);
+        else FB_EXACT_ROWS(64, false);
+    }
+#undef FB_EXACT_ROWS
     FB_HIP_CHECK(hipGetLastError());
     return FB_OK;
 }
@@ -2312,15 +2333,12 @@ int fast_pass_init()
         if (err == hipSuccess)   // the point keys' row counters
             err = hipFuncSetAttribute((const void *)k_point_keys_hist, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       kKeysBinsMax * (int)sizeof(uint32_t));
-        if (err == hipSuccess)   // the FP64 x lines' rings
-            err = hipFuncSetAttribute((const void *)k_exact_rows<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      fast_pass_max_lds());
-        if (err == hipSuccess)
-            err = hipFuncSetAttribute((const void *)k_exact_rows<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      fast_pass_max_lds());
-        if (err == hipSuccess)
-            err = hipFuncSetAttribute((const void *)k_exact_rows<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      fast_pass_max_lds());
+        // the FP64 x lines' rings
+        const void *rows_fns[6] = {(const void *)k_exact_rows<64, false>, (const void *)k_exact_rows<64, true>,
+                                   (const void *)k_exact_rows<32, false>, (const void *)k_exact_rows<32, true>,
+                                   (const void *)k_exact_rows<16, false>, (const void *)k_exact_rows<16, true>};
+        for (int i = 0; i < 6 && err == hipSuccess; ++i)
+            err = hipFuncSetAttribute(rows_fns[i], hipFuncAttributeMaxDynamicSharedMemorySize, fast_pass_max_lds());
     });
     if (err != hipSuccess) {
         (void)hipGetLastError();
